@@ -1,0 +1,298 @@
+/*
+ * smcrt.h — C ABI of the MI355X photon-packet engine (libsmcrt.so).
+ *
+ * The library replaces the body of signedMCRT's packet loop: the OpenMP region of
+ * `pathlength_scatter` (src/kernelsMod.f90:239-329) together with everything it calls per packet
+ * (`tauint2`/`update_grids`, src/inttau2.f90:15-441; `scatter`, src/photon.f90:713-779; the
+ * emitters, src/photon.f90:214-338,397-505; `reflect_refract`, src/surfaces.f90:14-120; the SDF
+ * `evaluate`/`calcNormal` stack under src/sdfs/).  It is plain C: fixed-width integers, doubles
+ * and pointers only, so Fortran binds it with `iso_c_binding` (signedmcrt_amd/fortran/smcrt.f90)
+ * and Python with ctypes (signedmcrt_amd/_abi.py).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: SMCRT_OK (0) or a negative SMCRT_E* code.  The text
+ *    of the last error on the calling thread is available from smcrt_last_error().  Nothing aborts
+ *    the process (the reference's `error stop` paths become status codes or per-packet
+ *    diagnostic counters, see smcrt_stats).
+ *  - Host buffers are caller-owned; the library owns device memory and copies results back
+ *    before returning.  smcrt_run_device() is the variant whose tally pointer is device memory.
+ *  - Fluence grids are column-major like the reference's `jmean(nxg,nyg,nzg)`
+ *    (src/iarray.f90:12): voxel (i,j,k), 0-based, lives at i + nx*(j + ny*k).
+ *  - A scene handle is not re-entrant: one caller thread per handle.
+ */
+#ifndef SMCRT_H
+#define SMCRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMCRT_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+    SMCRT_OK = 0,
+    SMCRT_EINVAL = -1,   /* malformed descriptor / argument            */
+    SMCRT_ENOMEM = -2,   /* host or device allocation failed           */
+    SMCRT_EDEVICE = -3,  /* HIP runtime error or no usable GPU         */
+    SMCRT_ERANGE = -4,   /* size beyond a documented limit             */
+    SMCRT_EIO = -5       /* file could not be read or written          */
+};
+
+/* ---- SDF program ------------------------------------------------------------------------
+ * A scene is the reference's `type(sdf)` array (src/sdfs/sdf_base.f90:29-39) flattened into a
+ * postfix program.  Each array element is a *region*: a contiguous node range whose evaluation
+ * leaves one signed distance.  Region order is array order; it decides the first-negative rule
+ * of the interface logic (src/inttau2.f90:131-140) and the layer chosen at emission
+ * (src/kernelsMod.f90:275-280).
+ *
+ * Node ops:
+ *  PRIM       push prim(q), q = [x y z 1]·T with T the 4x3 `t` (src/vector_class.f90:292-304)
+ *  CSG        d2 = pop, d1 = pop, push op(d1, d2, k); a `model` fold (src/sdfs/sdf_base.f90:145-160)
+ *             is child1 child2 CSG child3 CSG ...
+ *  MOD_BEGIN  push a modified evaluation position (twist, bend, revolution, elongate) or the
+ *             same position (extrude, onion) — src/sdfs/sdfModifiers.f90:273-392
+ *  MOD_END    pop that position and post-process the top distance (extrude, onion, elongate)
+ * Value-stack depth <= SMCRT_MAX_VALUE_STACK, position-stack depth <= SMCRT_MAX_POS_STACK.
+ */
+#define SMCRT_MAX_VALUE_STACK 4
+#define SMCRT_MAX_POS_STACK 3
+#define SMCRT_MAX_REGIONS 64
+#define SMCRT_MAX_NODES 4096
+#define SMCRT_MAX_MATERIALS 64
+
+enum { SMCRT_OP_PRIM = 1, SMCRT_OP_CSG = 2, SMCRT_OP_MOD_BEGIN = 3, SMCRT_OP_MOD_END = 4 };
+
+/* primitives (src/sdfs/sdfs.f90:494-735); parameters p[] in order: */
+enum {
+    SMCRT_SPHERE = 1,   /* radius                                                         */
+    SMCRT_BOX = 2,      /* half-lengths x,y,z (the constructor halves `lengths`, :510-525) */
+    SMCRT_TORUS = 3,    /* oradius, iradius                                               */
+    SMCRT_CYLINDER = 4, /* a.x a.y a.z b.x b.y b.z radius                                 */
+    SMCRT_TRIPRISM = 5, /* h1, h2                                                         */
+    SMCRT_SEGMENT = 6,  /* a.x a.y a.z b.x b.y b.z (tube thickness fixed at 0.1, :624)     */
+    SMCRT_CAPSULE = 7,  /* a.x a.y a.z b.x b.y b.z r                                       */
+    SMCRT_CONE = 8,     /* a.x a.y a.z b.x b.y b.z ra rb                                   */
+    SMCRT_EGG = 9,      /* r1, r2, h                                                      */
+    SMCRT_PLANE = 10    /* a.x a.y a.z (unit normal)                                       */
+};
+/* CSG operators (src/sdfs/sdfModifiers.f90:412-475) */
+enum { SMCRT_UNION = 1, SMCRT_SMOOTH_UNION = 2, SMCRT_SUBTRACTION = 3, SMCRT_INTERSECTION = 4 };
+/* modifiers (src/sdfs/sdfModifiers.f90:273-392); parameters p[]: */
+enum {
+    SMCRT_EXTRUDE = 1,    /* h            */
+    SMCRT_REVOLUTION = 2, /* o            */
+    SMCRT_ONION = 3,      /* thickness    */
+    SMCRT_ELONGATE = 4,   /* size x,y,z   */
+    SMCRT_TWIST = 5,      /* k            */
+    SMCRT_BEND = 6        /* k            */
+};
+
+typedef struct smcrt_node {
+    int32_t op;    /* SMCRT_OP_*                                                      */
+    int32_t kind;  /* primitive / CSG / modifier kind                                 */
+    double k;      /* CSG smoothing parameter (`model%k`)                             */
+    double p[8];   /* shape or modifier parameters                                    */
+    double t[12];  /* PRIM only: rows 1..4 of columns 1..3 of the 4x4 `transform`,
+                      row-major: q_j = x*t[j] + y*t[3+j] + z*t[6+j] + t[9+j]          */
+} smcrt_node;
+
+typedef struct smcrt_region {
+    int32_t first_node; /* index of the region's first node                          */
+    int32_t n_nodes;    /* number of nodes                                           */
+    int32_t material;   /* index into the material table                             */
+    int32_t layer;      /* the reference's `layer` id (informational)                */
+} smcrt_region;
+
+/* `mono(mus, mua, hgg, n)` (src/opticalProps/opticalProperties.f90:107-125).  The derived
+   kappa = mus + mua, albedo = mus/kappa (1 if mua < 1e-9) and g2 = hgg^2 are computed inside. */
+typedef struct smcrt_material {
+    double mus, mua, hgg, n;
+} smcrt_material;
+
+/* `cart_grid` (src/grid.f90:14-25, init_grid :53-93): counts and HALF sizes, centred at 0. */
+typedef struct smcrt_grid {
+    int32_t nx, ny, nz, pad;
+    double xmax, ymax, zmax;
+} smcrt_grid;
+
+/* sources (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338 */
+enum { SMCRT_SRC_PENCIL = 1, SMCRT_SRC_UNIFORM = 2, SMCRT_SRC_CIRCULAR = 3, SMCRT_SRC_POINT = 4 };
+
+typedef struct smcrt_source {
+    int32_t kind, pad;
+    double pos[3];   /* `position` (pencil, point, circular — the disc is centred at -pos)  */
+    double dir[3];   /* `direction` (pencil, uniform, circular)                           */
+    double p1[3];    /* uniform: pos = p1 + u*p2 + v*p3                                   */
+    double p2[3];
+    double p3[3];
+    double radius;   /* circular                                                          */
+} smcrt_source;
+
+typedef struct smcrt_scene_desc {
+    uint32_t abi_version; /* SMCRT_ABI_VERSION                                           */
+    int32_t n_nodes, n_regions, n_materials;
+    const smcrt_node* nodes;
+    const smcrt_region* regions;
+    const smcrt_material* materials;
+    smcrt_grid grid;
+    smcrt_source source;
+} smcrt_scene_desc;
+
+/* ---- run options, statistics, tallies -------------------------------------------------- */
+enum {
+    SMCRT_FLAG_MOMENTS = 1u << 0,   /* accumulate 1st/2nd moments of scatter sites 1..4
+                                       (the `test_kernel` harness, src/kernelsMod.f90:335-448) */
+    SMCRT_FLAG_END_EARLY = 1u << 1, /* terminate a packet at its 5th scatter (same harness)   */
+    SMCRT_FLAG_NO_TALLY = 1u << 2,  /* skip the fluence tally                                  */
+    SMCRT_FLAG_NO_LDS_TILE = 1u << 3 /* tally straight to HBM (no LDS-privatised tile)          */
+};
+
+typedef struct smcrt_run_opts {
+    uint64_t n_packets;    /* packets to launch (`nphotons`)                                 */
+    uint64_t seed;         /* RNG key (`iseed`)                                              */
+    uint64_t first_packet; /* global id of the first packet: packet j draws from stream
+                              (seed, first_packet + j), so results do not depend on launch
+                              shape and shards/CRN runs line up                              */
+    uint32_t flags;        /* SMCRT_FLAG_*                                                   */
+    int32_t max_bounces;   /* bounce cap, 0 -> 1000 (src/inttau2.f90:177)                    */
+    int32_t block_size;    /* 0 -> library default (64..1024, multiple of 64)                */
+    int32_t blocks_per_cu; /* 0 -> library default                                           */
+    int32_t chunk;         /* packets a wavefront claims per refill, 0 -> default            */
+    int32_t trace_packets; /* record the first `trace_packets` packets' events (0 = off)    */
+    int32_t trace_events;  /* events per traced packet                                       */
+    int32_t reserved[5];
+} smcrt_run_opts;
+
+/* How a packet ended.  A+R+T+K = launched. */
+typedef struct smcrt_stats {
+    uint64_t launched;
+    uint64_t absorbed;       /* albedo test failed (src/kernelsMod.f90:287-297)              */
+    uint64_t escaped_r;      /* left the world or the grid moving against the source dir     */
+    uint64_t escaped_t;      /* ... moving with it                                           */
+    uint64_t killed_surface; /* started a flight within eps of a surface (src/inttau2.f90:58) */
+    uint64_t killed_bounce;  /* bounce cap (src/inttau2.f90:177-180)                          */
+    uint64_t scatters;       /* `nscatt`                                                      */
+    uint64_t ev_sweeps;      /* full SDF sweeps (every region evaluated once)                 */
+    uint64_t ev_region_evals;/* region evaluations outside sweeps                             */
+    uint64_t ev_normals;     /* normal (gradient) evaluations                                 */
+    uint64_t ev_steps;       /* sphere-trace steps (each one a DDA walk)                      */
+    uint64_t ev_crossings;   /* voxel chords tallied                                          */
+    uint64_t ev_interfaces;  /* interface crossings resolved                                  */
+    uint64_t ev_fresnel;     /* interfaces with n1 != n2                                      */
+    uint64_t err_unreachable;/* the reference's "should not be reached" branch (:156)         */
+    uint64_t err_nan;        /* non-finite state caught and the packet dropped                */
+    double moments[2][4][3]; /* SMCRT_FLAG_MOMENTS: sum of pos and pos^2 at scatters 1..4    */
+    double kernel_ms;        /* device time of the packet kernel                             */
+    double tally_sum;        /* sum of the FP64 fluence accumulator (raw path length)        */
+} smcrt_stats;
+
+typedef struct smcrt_tallies {
+    double* jmean;     /* nx*ny*nz FP64 raw path-length sums (may be NULL)                    */
+    float* jmean_f32;  /* optional FP32 copy, the reference's `jmean` type (may be NULL)      */
+    float* trace;      /* optional: trace_packets*trace_events*4 floats (x,y,z,event)         */
+    smcrt_stats stats;
+} smcrt_tallies;
+
+typedef struct smcrt_scene smcrt_scene; /* opaque */
+
+/* ---- entry points ---------------------------------------------------------------------- */
+
+/* ABI version of the loaded library (SMCRT_ABI_VERSION). */
+int smcrt_abi_version(void);
+
+/* Copy the calling thread's last error message (NUL-terminated, truncated to len). */
+int smcrt_last_error(char* buf, size_t len);
+
+/* sizeof the named ABI struct ("node", "region", "material", "grid", "source", "scene_desc",
+   "run_opts", "stats", "tallies", "grad_out", "batch_set"), -1 if unknown.  No reference
+   counterpart: lets bindings (ctypes, iso_c_binding) check their struct layouts at load time. */
+int64_t smcrt_sizeof(const char* struct_name);
+
+/* Validate and flatten a scene.  Replaces `setup_simulation` + the SDF constructors
+   (src/setup.f90:14-54, src/sdfs/sdfs.f90:158-492).  Host work only; device memory is set up
+   on first use, so this succeeds without a GPU. */
+int smcrt_scene_create(const smcrt_scene_desc* desc, smcrt_scene** out);
+int smcrt_scene_destroy(smcrt_scene* scene);
+
+/* Replace the material table of an existing scene (same count).  Used for finite differences
+   and for gradient descent without re-flattening the geometry. */
+int smcrt_scene_set_materials(smcrt_scene* scene, const smcrt_material* materials, int32_t n);
+
+/* The packet loop.  Replaces the OpenMP region of `pathlength_scatter`
+   (src/kernelsMod.f90:239-329): launches opts->n_packets packets, accumulates the path-length
+   fluence estimator of `update_grids` (src/inttau2.f90:213-292) in FP64 and returns it in
+   out->jmean (raw, un-normalised; smcrt_normalise_fluence applies src/writer.f90:25-50). */
+int smcrt_run(smcrt_scene* scene, const smcrt_run_opts* opts, smcrt_tallies* out);
+
+/* Same, with the FP64 accumulator in device memory (d_jmean, nx*ny*nz doubles, added to, not
+   cleared) and `stream` a hipStream_t (NULL = default stream).  stats may be NULL.  Blocks until
+   the launch has finished. */
+int smcrt_run_device(smcrt_scene* scene, const smcrt_run_opts* opts, double* d_jmean,
+                     smcrt_stats* stats, void* stream);
+
+/* SDF evaluation on the device: the reference's `sdf%evaluate` (src/sdfs/sdf_base.f90:251-259)
+   and `calcNormal` (:165-189) for region `region` at n host points (xyz interleaved).
+   dist[n] and/or normals[3n] may be NULL.  FP32 on the device; normals are exact gradients
+   (forward-mode derivatives) rather than finite differences. */
+int smcrt_evaluate(smcrt_scene* scene, int32_t region, int64_t n, const float* points,
+                   float* dist, float* normals);
+
+/* ---- perturbation-MC adjoint (no reference counterpart) -------------------------------- */
+typedef struct smcrt_grad_out {
+    double* d_r_dmua;   /* n_materials: d(R_d)/d(mu_a) per material                          */
+    double* d_r_dmus;   /* n_materials: d(R_d)/d(mu_s)                                      */
+    double* d_w_dmua;   /* n_materials: d(sum_v w_v * raw_jmean_v / N)/d(mu_a)                */
+    double* d_w_dmus;   /* n_materials: same for mu_s                                       */
+    double r_d;         /* escaped_r / launched of this run                                 */
+    double w_tally;     /* sum_v w_v * raw_jmean_v / N of this run                           */
+    smcrt_stats stats;
+} smcrt_grad_out;
+
+/* Runs the packet loop with per-packet score accumulators (path length and scatter count per
+   material) and returns score-function derivatives of R_d and of the weighted tally
+   sum_v w_v*jmean_v (w = weights, nx*ny*nz floats on the host; NULL means w = 1, i.e. the
+   fluence sum).  jmean may be NULL. */
+int smcrt_grad(smcrt_scene* scene, const smcrt_run_opts* opts, const float* weights,
+               double* jmean, smcrt_grad_out* out);
+
+/* ---- batched launch (no reference counterpart) ----------------------------------------- */
+typedef struct smcrt_batch_set {
+    int32_t scene;       /* index into the scenes[] array of the batch                       */
+    int32_t pad;
+    uint64_t n_packets;  /* this set's budget                                                */
+    uint64_t first_packet;
+    const smcrt_material* materials; /* this set's material table (the scene's count)        */
+} smcrt_batch_set;
+
+typedef struct smcrt_batch smcrt_batch; /* opaque: device buffers + captured graph */
+
+/* Build a batch of n_sets property sets over n_scenes geometries.  Every set gets its own
+   FP64 grid (its scene's grid shape) and its own stats.  The launch is captured once into a
+   HIP graph and replayed by smcrt_batch_run. */
+int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt_batch_set* sets,
+                       int32_t n_sets, uint64_t seed, smcrt_batch** out);
+/* One persistent launch over every set (graph replay).  Clears the grids first. */
+int smcrt_batch_run(smcrt_batch* batch);
+/* Copy set `set`'s grid (raw FP64) and stats back; jmean may be NULL. */
+int smcrt_batch_read(smcrt_batch* batch, int32_t set, double* jmean, smcrt_stats* stats);
+int smcrt_batch_destroy(smcrt_batch* batch);
+
+/* ---- host-side output helpers (src/writer.f90) ----------------------------------------- */
+
+/* `normalise_fluence` (src/writer.f90:25-50): out = raw * (2x*2y)/(N*dx*dy*dz). */
+int smcrt_normalise_fluence(const smcrt_grid* grid, const double* raw, uint64_t n_packets,
+                            float* out);
+
+/* Write a 3-D FP32 array as NRRD the way `write_3d_r4_nrrd` does (src/writer.f90:220-251,
+   294-334): header, optional metadata lines (`meta`, may be NULL), blank line, raw data. */
+int smcrt_write_nrrd(const char* path, const float* data, int32_t nx, int32_t ny, int32_t nz,
+                     const char* meta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMCRT_H */

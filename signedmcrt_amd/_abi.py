@@ -1,0 +1,183 @@
+"""ctypes mirror of include/smcrt.h and the loader for libsmcrt.so.
+
+The structures here must match the C header field for field; tests/test_abi.py checks every
+size against the library's own `smcrt_sizeof_*` exports.  Loading fails loudly: there is no
+fallback path when the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsmcrt.so")
+
+ABI_VERSION = 1
+
+# status codes
+OK, EINVAL, ENOMEM, EDEVICE, ERANGE, EIO = 0, -1, -2, -3, -4, -5
+
+# node ops / kinds (smcrt.h)
+OP_PRIM, OP_CSG, OP_MOD_BEGIN, OP_MOD_END = 1, 2, 3, 4
+SPHERE, BOX, TORUS, CYLINDER, TRIPRISM, SEGMENT, CAPSULE, CONE, EGG, PLANE = range(1, 11)
+UNION, SMOOTH_UNION, SUBTRACTION, INTERSECTION = 1, 2, 3, 4
+EXTRUDE, REVOLUTION, ONION, ELONGATE, TWIST, BEND = range(1, 7)
+SRC_PENCIL, SRC_UNIFORM, SRC_CIRCULAR, SRC_POINT = 1, 2, 3, 4
+
+FLAG_MOMENTS = 1 << 0
+FLAG_END_EARLY = 1 << 1
+FLAG_NO_TALLY = 1 << 2
+FLAG_NO_LDS_TILE = 1 << 3
+
+MAX_REGIONS = 64
+MAX_NODES = 4096
+MAX_MATERIALS = 64
+
+
+class Node(C.Structure):
+    _fields_ = [("op", C.c_int32), ("kind", C.c_int32), ("k", C.c_double),
+                ("p", C.c_double * 8), ("t", C.c_double * 12)]
+
+
+class Region(C.Structure):
+    _fields_ = [("first_node", C.c_int32), ("n_nodes", C.c_int32),
+                ("material", C.c_int32), ("layer", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("mus", C.c_double), ("mua", C.c_double), ("hgg", C.c_double), ("n", C.c_double)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad", C.c_int32),
+                ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double)]
+
+
+class Source(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32),
+                ("pos", C.c_double * 3), ("dir", C.c_double * 3),
+                ("p1", C.c_double * 3), ("p2", C.c_double * 3), ("p3", C.c_double * 3),
+                ("radius", C.c_double)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("n_nodes", C.c_int32), ("n_regions", C.c_int32),
+                ("n_materials", C.c_int32),
+                ("nodes", C.POINTER(Node)), ("regions", C.POINTER(Region)),
+                ("materials", C.POINTER(Material)),
+                ("grid", Grid), ("source", Source)]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("n_packets", C.c_uint64), ("seed", C.c_uint64), ("first_packet", C.c_uint64),
+                ("flags", C.c_uint32), ("max_bounces", C.c_int32), ("block_size", C.c_int32),
+                ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
+                ("trace_packets", C.c_int32), ("trace_events", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+STAT_COUNTERS = ["launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",
+                 "killed_bounce", "scatters", "ev_sweeps", "ev_region_evals", "ev_normals",
+                 "ev_steps", "ev_crossings", "ev_interfaces", "ev_fresnel", "err_unreachable",
+                 "err_nan"]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in STAT_COUNTERS] + [
+        ("moments", C.c_double * 24), ("kernel_ms", C.c_double), ("tally_sum", C.c_double)]
+
+    def as_dict(self) -> dict:
+        d = {n: int(getattr(self, n)) for n in STAT_COUNTERS}
+        d["moments"] = [float(x) for x in self.moments]
+        d["kernel_ms"] = float(self.kernel_ms)
+        d["tally_sum"] = float(self.tally_sum)
+        return d
+
+
+class Tallies(C.Structure):
+    _fields_ = [("jmean", C.POINTER(C.c_double)), ("jmean_f32", C.POINTER(C.c_float)),
+                ("trace", C.POINTER(C.c_float)), ("stats", Stats)]
+
+
+class GradOut(C.Structure):
+    _fields_ = [("d_r_dmua", C.POINTER(C.c_double)), ("d_r_dmus", C.POINTER(C.c_double)),
+                ("d_w_dmua", C.POINTER(C.c_double)), ("d_w_dmus", C.POINTER(C.c_double)),
+                ("r_d", C.c_double), ("w_tally", C.c_double), ("stats", Stats)]
+
+
+class BatchSet(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("pad", C.c_int32), ("n_packets", C.c_uint64),
+                ("first_packet", C.c_uint64), ("materials", C.POINTER(Material))]
+
+
+STRUCTS = {"node": Node, "region": Region, "material": Material, "grid": Grid,
+           "source": Source, "scene_desc": SceneDesc, "run_opts": RunOpts, "stats": Stats,
+           "tallies": Tallies, "grad_out": GradOut, "batch_set": BatchSet}
+
+# every symbol include/smcrt.h declares (checked by tests/test_abi.py)
+EXPORTS = ["smcrt_abi_version", "smcrt_last_error", "smcrt_sizeof", "smcrt_scene_create", "smcrt_scene_destroy",
+           "smcrt_scene_set_materials", "smcrt_run", "smcrt_run_device", "smcrt_evaluate",
+           "smcrt_grad", "smcrt_batch_create", "smcrt_batch_run", "smcrt_batch_read",
+           "smcrt_batch_destroy", "smcrt_normalise_fluence", "smcrt_write_nrrd"]
+
+_lib = None
+
+
+class SmcrtError(RuntimeError):
+    """A non-zero status from libsmcrt, carrying smcrt_last_error()'s text."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"smcrt status {status}: {msg}")
+        self.status = status
+
+
+def lib() -> C.CDLL:
+    """The loaded libsmcrt.so (built by __graft_entry__.build()).  Raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -c "
+                          f"'import __graft_entry__ as g; g.build()'` (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "smcrt_abi_version": (C.c_int, []),
+        "smcrt_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "smcrt_scene_create": (C.c_int, [P(SceneDesc), P(C.c_void_p)]),
+        "smcrt_scene_destroy": (C.c_int, [C.c_void_p]),
+        "smcrt_scene_set_materials": (C.c_int, [C.c_void_p, P(Material), C.c_int32]),
+        "smcrt_run": (C.c_int, [C.c_void_p, P(RunOpts), P(Tallies)]),
+        "smcrt_run_device": (C.c_int, [C.c_void_p, P(RunOpts), C.c_void_p, P(Stats), C.c_void_p]),
+        "smcrt_evaluate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, P(C.c_float),
+                                     P(C.c_float), P(C.c_float)]),
+        "smcrt_grad": (C.c_int, [C.c_void_p, P(RunOpts), P(C.c_float), P(C.c_double), P(GradOut)]),
+        "smcrt_batch_create": (C.c_int, [P(C.c_void_p), C.c_int32, P(BatchSet), C.c_int32,
+                                         C.c_uint64, P(C.c_void_p)]),
+        "smcrt_batch_run": (C.c_int, [C.c_void_p]),
+        "smcrt_batch_read": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(Stats)]),
+        "smcrt_batch_destroy": (C.c_int, [C.c_void_p]),
+        "smcrt_normalise_fluence": (C.c_int, [P(Grid), P(C.c_double), C.c_uint64, P(C.c_float)]),
+        "smcrt_write_nrrd": (C.c_int, [C.c_char_p, P(C.c_float), C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_char_p]),
+        "smcrt_sizeof": (C.c_int64, [C.c_char_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if L.smcrt_abi_version() != ABI_VERSION:
+        raise ImportError("libsmcrt.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().smcrt_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise SmcrtError(status, last_error())

@@ -1,0 +1,359 @@
+// smcrt_device.h — device-side building blocks of the packet kernel (gfx950).
+//
+//  - float helpers on hardware instructions (v_sqrt_f32, v_rcp_f32, v_log_f32, v_sin/cos_f32 in
+//    revolutions) — the transport runs in FP32;
+//  - Dual: forward-mode derivative (value + gradient wrt position) so the SDF normal is the
+//    exact gradient of the SDF program instead of the reference's FP64 tetrahedral finite
+//    difference (src/sdfs/sdf_base.f90:165-189), whose h = 1e-6 is below FP32 resolution;
+//  - the SDF program interpreter, written once as a template over float and Dual;
+//  - Philox-4x32-10 keyed by (seed, packet id).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SMCRT_DEV __device__ __forceinline__
+
+// ------------------------------------------------------------------------------------------
+// device scene layout
+// ------------------------------------------------------------------------------------------
+// One SDF program node, 32 dwords so a wave-uniform index loads it with scalar loads.
+struct alignas(16) DevNode {
+    int op;       // SMCRT_OP_*
+    int kind;     // primitive / CSG / modifier kind
+    int ident;    // 1: PRIM transform is the identity (skip the 4x3 multiply)
+    float k;      // CSG smoothing
+    float p[8];
+    float t[12];
+    float pad[8];
+};
+static_assert(sizeof(DevNode) == 128, "DevNode must stay 32 dwords");
+
+struct DevRegion {
+    int first, count, material, pad;
+};
+
+// Per-material constants (one table per property set).
+struct alignas(16) DevMat {
+    float kappa, albedo, hgg, g2;
+    float n, mus, mua, pad;
+};
+
+// ------------------------------------------------------------------------------------------
+// float helpers
+// ------------------------------------------------------------------------------------------
+SMCRT_DEV float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+SMCRT_DEV float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+SMCRT_DEV float f_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+SMCRT_DEV float f_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+// -ln(u) via v_log_f32 (log2)
+SMCRT_DEV float f_neglog(float u) { return -0.69314718055994531f * __builtin_amdgcn_logf(u); }
+// sin/cos of 2*pi*r for r in revolutions (v_sin_f32 / v_cos_f32 take revolutions)
+SMCRT_DEV float f_sin_rev(float r) { return __builtin_amdgcn_sinf(r); }
+SMCRT_DEV float f_cos_rev(float r) { return __builtin_amdgcn_cosf(r); }
+SMCRT_DEV float f_sin(float x) { return __builtin_amdgcn_sinf(x * 0.15915494309189535f); }
+SMCRT_DEV float f_cos(float x) { return __builtin_amdgcn_cosf(x * 0.15915494309189535f); }
+
+SMCRT_DEV float val(float x) { return x; }
+
+// ------------------------------------------------------------------------------------------
+// Dual numbers: value and d/dx, d/dy, d/dz
+// ------------------------------------------------------------------------------------------
+struct Dual {
+    float v, x, y, z;
+};
+SMCRT_DEV float val(const Dual& a) { return a.v; }
+SMCRT_DEV Dual dconst(float c) { return Dual{c, 0.f, 0.f, 0.f}; }
+SMCRT_DEV Dual operator+(const Dual& a, const Dual& b) { return Dual{a.v + b.v, a.x + b.x, a.y + b.y, a.z + b.z}; }
+SMCRT_DEV Dual operator-(const Dual& a, const Dual& b) { return Dual{a.v - b.v, a.x - b.x, a.y - b.y, a.z - b.z}; }
+SMCRT_DEV Dual operator-(const Dual& a) { return Dual{-a.v, -a.x, -a.y, -a.z}; }
+SMCRT_DEV Dual operator+(const Dual& a, float b) { return Dual{a.v + b, a.x, a.y, a.z}; }
+SMCRT_DEV Dual operator-(const Dual& a, float b) { return Dual{a.v - b, a.x, a.y, a.z}; }
+SMCRT_DEV Dual operator-(float b, const Dual& a) { return Dual{b - a.v, -a.x, -a.y, -a.z}; }
+SMCRT_DEV Dual operator*(const Dual& a, float s) { return Dual{a.v * s, a.x * s, a.y * s, a.z * s}; }
+SMCRT_DEV Dual operator*(float s, const Dual& a) { return a * s; }
+SMCRT_DEV Dual operator*(const Dual& a, const Dual& b) {
+    return Dual{a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z};
+}
+SMCRT_DEV Dual operator/(const Dual& a, const Dual& b) {
+    float r = 1.0f / b.v;
+    float q = a.v * r;
+    return Dual{q, (a.x - q * b.x) * r, (a.y - q * b.y) * r, (a.z - q * b.z) * r};
+}
+SMCRT_DEV Dual operator/(const Dual& a, float b) { return a * (1.0f / b); }
+SMCRT_DEV Dual d_sqrt(const Dual& a) {
+    float s = sqrtf(a.v);
+    float h = s > 0.f ? 0.5f / s : 0.f;
+    return Dual{s, a.x * h, a.y * h, a.z * h};
+}
+SMCRT_DEV Dual d_abs(const Dual& a) { return a.v < 0.f ? -a : a; }
+SMCRT_DEV Dual d_min(const Dual& a, const Dual& b) { return b.v < a.v ? b : a; }
+SMCRT_DEV Dual d_max(const Dual& a, const Dual& b) { return b.v > a.v ? b : a; }
+SMCRT_DEV Dual d_sin(const Dual& a) { float s = sinf(a.v), c = cosf(a.v); return Dual{s, c * a.x, c * a.y, c * a.z}; }
+SMCRT_DEV Dual d_cos(const Dual& a) { float s = sinf(a.v), c = cosf(a.v); return Dual{c, -s * a.x, -s * a.y, -s * a.z}; }
+
+// generic wrappers so one template body serves float and Dual
+SMCRT_DEV float g_sqrt(float a) { return f_sqrt(a); }
+SMCRT_DEV Dual g_sqrt(const Dual& a) { return d_sqrt(a); }
+SMCRT_DEV float g_abs(float a) { return fabsf(a); }
+SMCRT_DEV Dual g_abs(const Dual& a) { return d_abs(a); }
+SMCRT_DEV float g_min(float a, float b) { return fminf(a, b); }
+SMCRT_DEV Dual g_min(const Dual& a, const Dual& b) { return d_min(a, b); }
+SMCRT_DEV float g_max(float a, float b) { return fmaxf(a, b); }
+SMCRT_DEV Dual g_max(const Dual& a, const Dual& b) { return d_max(a, b); }
+SMCRT_DEV float g_max0(float a) { return fmaxf(a, 0.f); }
+SMCRT_DEV Dual g_max0(const Dual& a) { return a.v > 0.f ? a : dconst(0.f); }
+SMCRT_DEV float g_min0(float a) { return fminf(a, 0.f); }
+SMCRT_DEV Dual g_min0(const Dual& a) { return a.v < 0.f ? a : dconst(0.f); }
+SMCRT_DEV float g_clamp01(float a) { return fminf(fmaxf(a, 0.f), 1.f); }
+SMCRT_DEV Dual g_clamp01(const Dual& a) { return a.v < 0.f ? dconst(0.f) : (a.v > 1.f ? dconst(1.f) : a); }
+SMCRT_DEV float g_div(float a, float b) { return f_div(a, b); }
+SMCRT_DEV Dual g_div(const Dual& a, const Dual& b) { return a / b; }
+SMCRT_DEV float g_sin(float a) { return f_sin(a); }
+SMCRT_DEV Dual g_sin(const Dual& a) { return d_sin(a); }
+SMCRT_DEV float g_cos(float a) { return f_cos(a); }
+SMCRT_DEV Dual g_cos(const Dual& a) { return d_cos(a); }
+template <class T> SMCRT_DEV T g_const(float c);
+template <> SMCRT_DEV float g_const<float>(float c) { return c; }
+template <> SMCRT_DEV Dual g_const<Dual>(float c) { return dconst(c); }
+// sign(s) * a with s's sign as a float multiplier
+SMCRT_DEV float g_neg_if(float a, bool neg) { return neg ? -a : a; }
+SMCRT_DEV Dual g_neg_if(const Dual& a, bool neg) { return neg ? -a : a; }
+
+template <class T>
+struct V3 {
+    T x, y, z;
+};
+template <class T> SMCRT_DEV T vdot(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> SMCRT_DEV V3<T> vsub(const V3<T>& a, const V3<T>& b) { return V3<T>{a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> SMCRT_DEV V3<T> vsubc(const V3<T>& a, float bx, float by, float bz) { return V3<T>{a.x - bx, a.y - by, a.z - bz}; }
+template <class T> SMCRT_DEV V3<T> vabs(const V3<T>& a) { return V3<T>{g_abs(a.x), g_abs(a.y), g_abs(a.z)}; }
+template <class T> SMCRT_DEV T vlen(const V3<T>& a) { return g_sqrt(vdot(a, a)); }
+template <class T> SMCRT_DEV V3<T> vmax0(const V3<T>& a) { return V3<T>{g_max0(a.x), g_max0(a.y), g_max0(a.z)}; }
+
+// ------------------------------------------------------------------------------------------
+// SDF primitives (src/sdfs/sdfs.f90:494-735)
+// ------------------------------------------------------------------------------------------
+template <class T>
+SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p) {
+    switch (kind) {
+    case 1: {  // sphere
+        return g_sqrt(vdot(p, p)) - P[0];
+    }
+    case 2: {  // box
+        V3<T> q = vsubc(vabs(p), P[0], P[1], P[2]);
+        return vlen(vmax0(q)) + g_min0(g_max(q.x, g_max(q.y, q.z)));
+    }
+    case 3: {  // torus
+        T a = g_sqrt(p.x * p.x + p.z * p.z) - P[0];
+        return g_sqrt(a * a + p.y * p.y) - P[1];
+    }
+    case 4: {  // cylinder(a, b, r)
+        float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
+        float baba = bax * bax + bay * bay + baz * baz;
+        V3<T> pa = vsubc(p, P[0], P[1], P[2]);
+        T paba = pa.x * bax + pa.y * bay + pa.z * baz;
+        V3<T> w{pa.x * baba - paba * bax, pa.y * baba - paba * bay, pa.z * baba - paba * baz};
+        T x = vlen(w) - P[6] * baba;
+        T y = g_abs(paba - baba * 0.5f) - baba * 0.5f;
+        T x2 = x * x;
+        T y2 = (y * y) * baba;
+        float xv = val(x), yv = val(y);
+        T d;
+        if (fmaxf(xv, yv) < 0.f) d = -g_min(x2, y2);
+        else if (xv > 0.f && yv > 0.f) d = x2 + y2;
+        else if (xv > 0.f) d = x2;
+        else if (yv > 0.f) d = y2;
+        else d = g_const<T>(0.f);
+        T r = g_sqrt(g_abs(d)) * (1.0f / baba);
+        return g_neg_if(r, val(d) < 0.f);
+    }
+    case 5: {  // triprism(h1, h2)
+        V3<T> q = vabs(p);
+        return g_max(q.z - P[1], g_max(q.x * 0.866025f + p.y * 0.5f, -p.y) - P[0] * 0.5f);
+    }
+    case 6:    // segment (tube thickness 0.1)
+    case 7: {  // capsule(a, b, r)
+        float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
+        float baba = bax * bax + bay * bay + baz * baz;
+        V3<T> pa = vsubc(p, P[0], P[1], P[2]);
+        T h = g_clamp01((pa.x * bax + pa.y * bay + pa.z * baz) * (1.0f / baba));
+        V3<T> w{pa.x - h * bax, pa.y - h * bay, pa.z - h * baz};
+        return vlen(w) - (kind == 6 ? 0.1f : P[6]);
+    }
+    case 8: {  // capped cone(a, b, ra, rb)
+        float ra = P[6], rb = P[7], rba = rb - ra;
+        float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
+        float baba = bax * bax + bay * bay + baz * baz;
+        V3<T> pa = vsubc(p, P[0], P[1], P[2]);
+        T papa = vdot(pa, pa);
+        T paba = (pa.x * bax + pa.y * bay + pa.z * baz) * (1.0f / baba);
+        T x = g_sqrt(g_max0(papa - baba * paba * paba));
+        T cax = val(paba) < 0.5f ? g_max0(x - ra) : g_max0(x - rb);
+        T cay = g_abs(paba - 0.5f) - 0.5f;
+        float k = rba * rba + baba;
+        T f = g_clamp01((rba * (x - ra) + paba * baba) * (1.0f / k));
+        T cbx = x - ra - f * rba;
+        T cby = paba - f;
+        bool neg = val(cbx) < 0.f && val(cay) < 0.f;
+        T r = g_sqrt(g_min(cax * cax + baba * (cay * cay), cbx * cbx + baba * (cby * cby)));
+        return g_neg_if(r, neg);
+    }
+    case 9: {  // egg(r1, r2, h)
+        float r1 = P[0], r2 = P[1], h = P[2];
+        float r = r1 - r2, hin = h + r;
+        float l = (hin * hin - r * r) / (2.f * r);
+        T px = g_abs(p.x);
+        if (val(p.y) <= 0.f) return g_sqrt(px * px + p.y * p.y + p.z * p.z) - r1;
+        if ((val(p.y) - hin) * l > val(px) * hin) {
+            T dy = p.y - hin;
+            return g_sqrt(px * px + dy * dy + p.z * p.z) - ((r1 + l) - sqrtf(hin * hin + l * l));
+        }
+        T dx = px + l;
+        return g_sqrt(dx * dx + p.y * p.y + p.z * p.z) - (r1 + l);
+    }
+    case 10: {  // plane(a)
+        return p.x * P[0] + p.y * P[1] + p.z * P[2];
+    }
+    }
+    return g_const<T>(0.f);
+}
+
+// CSG operators (src/sdfs/sdfModifiers.f90:412-475)
+template <class T>
+SMCRT_DEV T csg_eval(int kind, const T& d1, const T& d2, float k) {
+    switch (kind) {
+    case 1: return g_min(d1, d2);
+    case 2: {
+        T h = g_max0(k - g_abs(d1 - d2)) * (1.0f / k);
+        return g_min(d1, d2) - h * h * h * (k * (1.0f / 6.0f));
+    }
+    case 3: return g_max(-d1, d2);
+    case 4: return g_max(d1, d2);
+    }
+    return d1;
+}
+
+// modifier position maps (src/sdfs/sdfModifiers.f90:290-375)
+template <class T>
+SMCRT_DEV V3<T> mod_pos(int kind, const float* __restrict__ P, const V3<T>& p) {
+    switch (kind) {
+    case 2: {  // revolution
+        T a = g_sqrt(p.x * p.x + p.z * p.z) - P[0];
+        return V3<T>{a, p.y, g_const<T>(0.f)};
+    }
+    case 4:  // elongate
+        return vmax0(vsubc(vabs(p), P[0], P[1], P[2]));
+    case 5: {  // twist
+        T a = p.z * P[0];
+        T c = g_cos(a), s = g_sin(a);
+        return V3<T>{c * p.x - s * p.y, s * p.x + c * p.y, p.z};
+    }
+    case 6: {  // bend
+        T a = p.x * P[0];
+        T c = g_cos(a), s = g_sin(a);
+        return V3<T>{c * p.x - s * p.y, s * p.x + c * p.y, p.z};
+    }
+    }
+    return p;  // extrude, onion
+}
+
+// modifier post-processing (src/sdfs/sdfModifiers.f90:273-335)
+template <class T>
+SMCRT_DEV T mod_post(int kind, const float* __restrict__ P, const V3<T>& p, const T& d) {
+    switch (kind) {
+    case 1: {  // extrude
+        T wy = g_abs(p.z) - P[0];
+        T mx = g_max0(d), my = g_max0(wy);
+        return g_min0(g_max(d, wy)) + g_sqrt(mx * mx + my * my);
+    }
+    case 3:  // onion
+        return g_abs(d) - P[0];
+    case 4: {  // elongate
+        V3<T> q = vsubc(vabs(p), P[0], P[1], P[2]);
+        return d + g_min0(g_max(q.x, g_max(q.y, q.z)));
+    }
+    }
+    return d;
+}
+
+template <class T>
+SMCRT_DEV V3<T> xform(const float* __restrict__ t, const V3<T>& p) {
+    return V3<T>{p.x * t[0] + p.y * t[3] + p.z * t[6] + t[9],
+                 p.x * t[1] + p.y * t[4] + p.z * t[7] + t[10],
+                 p.x * t[2] + p.y * t[5] + p.z * t[8] + t[11]};
+}
+
+// Evaluate one region's postfix program at p.  Node range and node contents are wave-uniform,
+// so the opcode dispatch is a scalar branch and parameters come from scalar loads.  The value
+// and position stacks are shift registers (no dynamically indexed register arrays).
+template <class T>
+SMCRT_DEV T region_eval(const DevNode* __restrict__ nodes, int first, int count, const V3<T>& p0) {
+    T v0 = g_const<T>(0.f), v1 = v0, v2 = v0, v3 = v0;
+    V3<T> q0 = p0, q1 = p0, q2 = p0;  // q0 = top of the position stack
+    for (int i = 0; i < count; ++i) {
+        const DevNode* n = nodes + first + i;
+        const int op = __builtin_amdgcn_readfirstlane(n->op);
+        const int kind = __builtin_amdgcn_readfirstlane(n->kind);
+        if (op == 1) {  // PRIM
+            V3<T> q = n->ident ? q0 : xform<T>(n->t, q0);
+            T d = prim_eval<T>(kind, n->p, q);
+            v3 = v2; v2 = v1; v1 = v0; v0 = d;
+        } else if (op == 2) {  // CSG: d1 = v1, d2 = v0
+            T d = csg_eval<T>(kind, v1, v0, n->k);
+            v0 = d; v1 = v2; v2 = v3;
+        } else if (op == 3) {  // MOD_BEGIN
+            V3<T> q = mod_pos<T>(kind, n->p, q0);
+            q2 = q1; q1 = q0; q0 = q;
+        } else {  // MOD_END
+            q0 = q1; q1 = q2;
+            v0 = mod_post<T>(kind, n->p, q0, v0);
+        }
+    }
+    return v0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon et al., SC'11).  Counter (block, id_lo, id_hi, 0), key (seed_lo, seed_hi).
+// ------------------------------------------------------------------------------------------
+struct Rng {
+    uint32_t id_lo, id_hi, block, left;
+    uint32_t b0, b1, b2, b3;
+};
+
+SMCRT_DEV void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+SMCRT_DEV void rng_init(Rng& g, uint64_t id) {
+    g.id_lo = (uint32_t)id;
+    g.id_hi = (uint32_t)(id >> 32);
+    g.block = 0;
+    g.left = 0;
+}
+
+// uniform in (0,1): odd multiple of 2^-24, bit-identical to the oracle's draw
+SMCRT_DEV float rng_u(Rng& g, uint32_t k0, uint32_t k1) {
+    if (g.left == 0) {
+        uint32_t c0 = g.block, c1 = g.id_lo, c2 = g.id_hi, c3 = 0u;
+        philox10(c0, c1, c2, c3, k0, k1);
+        g.b0 = c0; g.b1 = c1; g.b2 = c2; g.b3 = c3;
+        g.block++;
+        g.left = 4;
+    }
+    uint32_t x = g.b0;
+    g.b0 = g.b1; g.b1 = g.b2; g.b2 = g.b3;
+    g.left--;
+    return (float)(((x >> 9) << 1) | 1u) * (1.0f / 16777216.0f);
+}

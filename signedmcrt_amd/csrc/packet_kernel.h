@@ -52,7 +52,8 @@ struct KParams {
     float spos[3], sdir[3], p1[3], p2[3], p3[3], radius;
     float circ_t[12];
     float rdir[3];  // R/T classification direction
-    float eps;
+    float eps;       // march / crossing epsilon
+    float eps_kill;  // flight-start kill threshold after emission or scatter
     // run
     uint32_t seed_lo, seed_hi;
     unsigned long long total;          // work items (sum of set budgets)
@@ -78,6 +79,9 @@ struct Lane {
     float opx, opy, opz;  // position before the crossing step
     float tau, taurun, dlay_old;
     int layer, st, bounces, nsc, set, tcount;
+    float dprev;     // distance at the previous march step
+    bool iface;      // the current flight started at an interface
+    unsigned iters;
     unsigned long long pid;
     Rng rng;
 };
@@ -300,6 +304,9 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     L.set = 0;
     L.layer = 0;
     L.tcount = 0;
+    L.iface = false;
+    L.dprev = 0.f;
+    L.iters = 0;
     unsigned out[NOUT];
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) out[i] = 0;
@@ -311,7 +318,8 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     unsigned long long base = 0;
     unsigned avail = 0;
     bool exhausted = false;
-    const float eps = P.eps;
+    const float eps = P.eps;          // march / crossing epsilon (FP32-safe)
+    const float eps_kill = P.eps_kill; // flight-start kill threshold after emission or scatter
     const unsigned chunk = (unsigned)P.chunk;
     const int lane_id = __lane_id();
 
@@ -375,6 +383,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             L.bounces = 0;
             L.nsc = 0;
             L.tcount = 0;
+            L.iters = 0;
             out[O_LAUNCHED]++;
             if (ADJ) {
                 adjC = 0.f;
@@ -393,60 +402,50 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             for (int r = 0; r < P.nreg; ++r) {
                 const DevRegion reg = P.regions[r];
                 float d = region_eval<float>(P.nodes, reg.first, reg.count, p);
-                float a = fabsf(d);
-                dmin = fminf(dmin, a);
+                dmin = fminf(dmin, fabsf(d));
                 smin = fminf(smin, d);
                 if (r == 0) d0 = d;
                 if (fn < 0 && d < 0.f) { fn = r; dfirst = d; }
                 if (r == L.layer) dlay = d;
-                float c = d > 0.f ? 999.f : -d;
+                float c = d > 0.f ? 999.f : -d;  // positive distances count as -999
                 if (c < bestv) { bestv = c; best = r; }
             }
             ev[E_SWEEPS]++;
-            if (fn < 0) { fn = 0; dfirst = d0; }
-        }
-
-        int end_kind = -1;       // >= 0: the packet ends this iteration
-        bool want_dstmp = false; // needs region fn evaluated at the pre-crossing position
-        bool want_normal = false;
-        int use_region = 0;
-
-        if (L.st == ST_EMIT) {
-            L.layer = best;
-            dlay = (best == 0) ? d0 : -bestv;  // value at the chosen region (<= 0 when inside)
-            L.tau = f_neglog(rnd(L, P));
-            L.taurun = 0.f;
-            L.st = ST_START;
-        }
-
-        // march decision for START / MARCHED
-        bool do_march = false;
-        float dstep = 0.f;
-        if (L.st == ST_START) {
-            if (dmin < eps) end_kind = END_SURFACE;
-            else if (dmin > eps) { do_march = true; dstep = dmin; }
-            else L.st = ST_MARCHED;  // d == eps: straight to the interface below
-        }
-        if (L.st == ST_MARCHED && !do_march && end_kind < 0) {
-            if (smin >= 0.f) {
-                end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
-            } else if (dmin > eps) {
-                do_march = true;
-                dstep = dmin;
-            } else {
-                // interface: step a bit past the surface (src/inttau2.f90:104-116)
-                ev[E_IFACE]++;
-                L.opx = L.px; L.opy = L.py; L.opz = L.pz;
-                L.dlay_old = dlay;
-                float s = dmin + 2.f * eps;
-                L.px += s * L.dx; L.py += s * L.dy; L.pz += s * L.dz;
-                L.st = ST_CROSSED;
-                goto next_iter_state;
+            if (fn < 0) { fn = 0; dfirst = d0; }  // minloc of an all-zero array is index 1
+            if (++L.iters > (1u << 24)) {         // runaway guard: never expected
+                ev[E_NAN]++;
+                L.st = ST_IDLE;
             }
         }
 
-        if (L.st == ST_CROSSED && end_kind < 0 && !do_march) {
-            // the sweep just ran at the crossed position
+        if (L.st == ST_EMIT) {
+            L.layer = best;  // nearest containing surface (src/kernelsMod.f90:275-280)
+            L.tau = f_neglog(rnd(L, P));
+            L.taurun = 0.f;
+            L.iface = false;
+            L.st = ST_START;
+        }
+
+        int end_kind = -1;  // >= 0: the packet ends this iteration
+        bool want_dstmp = false, want_normal = false, do_march = false, cross = false;
+        int use_region = 0;
+        const int st0 = L.st;
+        if (st0 == ST_START) {
+            // src/inttau2.f90:52-63.  After an interface the threshold is the march epsilon (the
+            // crossing-step ratio that kills grazing crossings and reflected packets); after
+            // emission or a scatter it is the reference's own 1e-8.
+            const float thr = L.iface ? eps : eps_kill;
+            if (dmin < thr) end_kind = END_SURFACE;
+            else if (dmin > eps || !L.iface) do_march = true;
+            else cross = true;  // exactly eps after an interface: straight to the crossing
+        } else if (st0 == ST_MARCHED) {
+            // src/inttau2.f90:85-97
+            if (smin >= 0.f) end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
+            else if (dmin > eps) do_march = true;
+            else if (dmin < L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
+            else do_march = true;  // within eps but moving away from the surface: keep marching
+        } else if (st0 == ST_CROSSED) {
+            // the sweep ran at the crossed position (src/inttau2.f90:117-162)
             const int nl = fn;
             const float n1 = mt[L.layer].n, n2 = mt[nl].n;
             if (n1 != n2) {
@@ -456,12 +455,11 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     use_region = fabsf(dn) < fabsf(dl) ? nl : L.layer;
                     want_normal = true;
                 } else if (L.dlay_old < 0.f) {
-                    // may need dstmp(new) = region nl at the pre-crossing position
                     if (nl == L.layer) {
-                        use_region = L.layer;  // maxloc of two equal values -> first
+                        use_region = L.layer;  // maxloc of two equal values -> the first
                         want_normal = true;
                     } else {
-                        want_dstmp = true;
+                        want_dstmp = true;  // needs region nl at the pre-crossing position
                     }
                 } else if (dl > 0.f && dn < 0.f) {
                     use_region = L.layer;
@@ -473,19 +471,29 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     end_kind = END_ERR;
                 }
             } else {
-                L.layer = nl;
+                L.layer = nl;  // index-matched: switch region, keep tau (:182-184)
+                L.iface = true;
                 L.st = ST_START;
             }
         }
 
-        // dstmp(new) evaluations (waterfall over the distinct regions)
+        if (cross) {
+            // interface: step a little past the surface (src/inttau2.f90:104-116)
+            ev[E_IFACE]++;
+            L.opx = L.px; L.opy = L.py; L.opz = L.pz;
+            L.dlay_old = dlay;
+            const float s = dmin + 2.f * eps;
+            L.px += s * L.dx; L.py += s * L.dy; L.pz += s * L.dz;
+            L.st = ST_CROSSED;
+        }
+
+        // dstmp(new): region nl at the pre-crossing position (waterfall over distinct regions)
         if (wave_ballot(want_dstmp)) {
             V3<float> op{L.opx, L.opy, L.opz};
             float dt = eval_lane_region<float>(P, want_dstmp, fn, op);
             if (want_dstmp) {
                 ev[E_REGION]++;
                 const float dl = dlay, dn = dfirst;
-                want_dstmp = false;
                 if (dt < 0.f) {
                     use_region = dt > L.dlay_old ? fn : L.layer;
                     want_normal = true;
@@ -501,7 +509,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             }
         }
 
-        // normals + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
+        // normal + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
         if (wave_ballot(want_normal)) {
             V3<Dual> pd{Dual{L.px, 1.f, 0.f, 0.f}, Dual{L.py, 0.f, 1.f, 0.f}, Dual{L.pz, 0.f, 0.f, 1.f}};
             Dual g = eval_lane_region<Dual>(P, want_normal, use_region, pd);
@@ -512,7 +520,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 const int nl = fn;
                 const float n1 = mt[L.layer].n, n2 = mt[nl].n;
                 float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);
-                bool rflag = rnd(L, P) <= R;
+                bool rflag = rnd(L, P) <= R;  // draw: reflect or refract
                 if (rflag) {
                     float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
                     L.dx -= c * nx; L.dy -= c * ny; L.dz -= c * nz;
@@ -525,13 +533,14 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     float k = eta * c1 - c2;
                     L.dx = eta * L.dx + k * nx; L.dy = eta * L.dy + k * ny; L.dz = eta * L.dz + k * nz;
                 }
-                L.tau = f_neglog(rnd(L, P));
+                L.tau = f_neglog(rnd(L, P));  // draw: resampled optical depth (:168)
                 L.taurun = 0.f;
+                L.iface = true;
                 if (!rflag) {
                     L.layer = nl;
                     trace_event(P, L, 3);
                 } else {
-                    L.px = L.opx; L.py = L.opy; L.pz = L.opz;
+                    L.px = L.opx; L.py = L.opy; L.pz = L.opz;  // step back inside (:174)
                     trace_event(P, L, 4);
                     if (++L.bounces > P.max_bounces) end_kind = END_BOUNCE;
                 }
@@ -542,20 +551,21 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         // ---- march step (src/inttau2.f90:64-83) ----
         if (do_march) {
             ev[E_STEPS]++;
+            L.dprev = dmin;
             const float kap = mt[L.layer].kappa;
-            const float t = dstep * kap;
-            bool flight_over = !(L.taurun + t <= L.tau);
-            float d = dstep;
+            const float t = dmin * kap;
+            const bool flight_over = !(L.taurun + t <= L.tau);
+            float d = dmin;
             if (flight_over) d = (L.tau - L.taurun) / kap;
             else L.taurun += t;
             float in_grid = 0.f, aint = 0.f;
-            bool inside = dda<ADJ>(P, grid, L, d, &in_grid, ev[E_CROSS], &adjC, &aint);
+            const bool inside = dda<ADJ>(P, grid, L, d, &in_grid, ev[E_CROSS], &adjC, &aint);
             if (ADJ) {
                 const int m = P.region_mat[L.layer];
                 if (inside) {
                     adj_sc(A, 0, m, P.nmat) += d;
-                    adj_sc(A, 1, m, P.nmat) += aint + adjC * (d - in_grid);
-                } else {
+                    adj_sc(A, 1, m, P.nmat) += aint;
+                } else {  // left the grid: the score stops at the exit point
                     adj_sc(A, 0, m, P.nmat) += in_grid;
                     adj_sc(A, 1, m, P.nmat) += aint;
                 }
@@ -585,8 +595,9 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     if ((P.flags & 2u) && L.nsc >= 5) {
                         end_kind = END_ABSORB;
                     } else {
-                        L.tau = f_neglog(rnd(L, P));
+                        L.tau = f_neglog(rnd(L, P));  // draw: next flight's optical depth
                         L.taurun = 0.f;
+                        L.iface = false;
                         L.st = ST_START;
                     }
                 } else {
@@ -597,7 +608,6 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             }
         }
 
-    next_iter_state:
         // non-finite guard
         if (end_kind < 0 && L.st != ST_IDLE && !(fabsf(L.px) < 1e30f && fabsf(L.py) < 1e30f && fabsf(L.pz) < 1e30f)) {
             ev[E_NAN]++;

@@ -1,0 +1,908 @@
+// smcrt_api.hip — the C ABI of include/smcrt.h: scene flattening/validation, device set-up, the
+// packet-loop launches (single scene, adjoint, batched + HIP graph), SDF evaluation and the
+// host-side output helpers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/smcrt.h"
+#include "packet_kernel.h"
+
+using namespace smcrt;
+
+// ------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHECK(x)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) return fail(SMCRT_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------
+// small device kernels
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_u64(unsigned long long* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+__global__ void k_fill_f64(double* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+__global__ void k_sum_f64(const double* __restrict__ p, size_t n, double* out) {
+    __shared__ double part[256];
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += p[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+// SDF values / exact gradients of one region at n points (one point per thread; the region's
+// program is uniform across the grid).
+__global__ void k_evaluate(const DevNode* __restrict__ nodes, DevRegion reg, long long n,
+                           const float* __restrict__ pts, float* __restrict__ dist,
+                           float* __restrict__ nrm) {
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    if (dist) dist[i] = region_eval<float>(nodes, reg.first, reg.count, V3<float>{x, y, z});
+    if (nrm) {
+        V3<Dual> p{Dual{x, 1.f, 0.f, 0.f}, Dual{y, 0.f, 1.f, 0.f}, Dual{z, 0.f, 0.f, 1.f}};
+        Dual g = region_eval<Dual>(nodes, reg.first, reg.count, p);
+        float inv = 1.0f / sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+        nrm[3 * i] = g.x * inv;
+        nrm[3 * i + 1] = g.y * inv;
+        nrm[3 * i + 2] = g.z * inv;
+    }
+}
+
+static inline size_t fill_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    return b < 4096 ? (b ? b : 1) : 4096;
+}
+
+// ------------------------------------------------------------------------------------------
+// scene
+// ------------------------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return SMCRT_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (hipMalloc(&p, want) != hipSuccess) return fail(SMCRT_ENOMEM, "hipMalloc(%zu) failed", want);
+        bytes = want;
+        return SMCRT_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct smcrt_scene {
+    std::vector<DevNode> nodes;
+    std::vector<DevRegion> regions;
+    std::vector<int> region_mat;
+    std::vector<smcrt_material> mats;
+    smcrt_grid grid;
+    smcrt_source src;
+    float eps, eps_kill;
+    float circ_t[12];
+    int circ_swap;
+    float rdir[3];
+    // device
+    int device = -1;
+    int num_cu = 0;
+    bool uploaded = false;
+    bool mats_dirty = true;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
+        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum;
+};
+
+static int n_values_pushed(int op) { return op == SMCRT_OP_PRIM ? 1 : (op == SMCRT_OP_CSG ? -1 : 0); }
+
+static bool finite_all(const double* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+static void region_mats(const smcrt_scene* s, const smcrt_material* mats, DevMat* out) {
+    for (size_t r = 0; r < s->regions.size(); ++r) {
+        const smcrt_material& m = mats[s->region_mat[r]];
+        DevMat d;
+        d.kappa = (float)(m.mus + m.mua);
+        d.albedo = (m.mua < 1e-9) ? 1.0f : (float)(m.mus / (m.mus + m.mua));
+        d.hgg = (float)m.hgg;
+        d.g2 = (float)(m.hgg * m.hgg);
+        d.n = (float)m.n;
+        d.mus = (float)m.mus;
+        d.mua = (float)m.mua;
+        d.pad = 0.f;
+        out[r] = d;
+    }
+}
+
+static int validate_materials(const smcrt_material* m, int n) {
+    for (int i = 0; i < n; ++i) {
+        const smcrt_material& x = m[i];
+        if (!(std::isfinite(x.mus) && std::isfinite(x.mua) && std::isfinite(x.hgg) && std::isfinite(x.n)))
+            return fail(SMCRT_EINVAL, "material %d has a non-finite property", i);
+        if (x.mus < 0 || x.mua < 0) return fail(SMCRT_EINVAL, "material %d: mus and mua must be >= 0", i);
+        if (!(x.hgg > -1.0 && x.hgg < 1.0)) return fail(SMCRT_EINVAL, "material %d: hgg must lie in (-1, 1)", i);
+        if (!(x.n > 0)) return fail(SMCRT_EINVAL, "material %d: refractive index must be > 0", i);
+    }
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_abi_version(void) { return SMCRT_ABI_VERSION; }
+
+extern "C" int smcrt_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return SMCRT_EINVAL;
+    snprintf(buf, len, "%s", g_err.c_str());
+    return SMCRT_OK;
+}
+
+extern "C" int64_t smcrt_sizeof(const char* name) {
+    if (!name) return -1;
+    struct { const char* n; int64_t s; } t[] = {
+        {"node", sizeof(smcrt_node)}, {"region", sizeof(smcrt_region)},
+        {"material", sizeof(smcrt_material)}, {"grid", sizeof(smcrt_grid)},
+        {"source", sizeof(smcrt_source)}, {"scene_desc", sizeof(smcrt_scene_desc)},
+        {"run_opts", sizeof(smcrt_run_opts)}, {"stats", sizeof(smcrt_stats)},
+        {"tallies", sizeof(smcrt_tallies)}, {"grad_out", sizeof(smcrt_grad_out)},
+        {"batch_set", sizeof(smcrt_batch_set)}};
+    for (auto& e : t)
+        if (!strcmp(e.n, name)) return e.s;
+    return -1;
+}
+
+extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) {
+    if (!d || !out) return fail(SMCRT_EINVAL, "null descriptor or output handle");
+    *out = nullptr;
+    if (d->abi_version != SMCRT_ABI_VERSION)
+        return fail(SMCRT_EINVAL, "abi_version %u, library speaks %d", d->abi_version, SMCRT_ABI_VERSION);
+    if (d->n_regions < 1 || d->n_regions > SMCRT_MAX_REGIONS)
+        return fail(SMCRT_ERANGE, "n_regions %d outside [1, %d]", d->n_regions, SMCRT_MAX_REGIONS);
+    if (d->n_nodes < 1 || d->n_nodes > SMCRT_MAX_NODES)
+        return fail(SMCRT_ERANGE, "n_nodes %d outside [1, %d]", d->n_nodes, SMCRT_MAX_NODES);
+    if (d->n_materials < 1 || d->n_materials > SMCRT_MAX_MATERIALS)
+        return fail(SMCRT_ERANGE, "n_materials %d outside [1, %d]", d->n_materials, SMCRT_MAX_MATERIALS);
+    if (!d->nodes || !d->regions || !d->materials) return fail(SMCRT_EINVAL, "null nodes/regions/materials");
+    int rc = validate_materials(d->materials, d->n_materials);
+    if (rc) return rc;
+    const smcrt_grid& g = d->grid;
+    if (g.nx < 1 || g.ny < 1 || g.nz < 1) return fail(SMCRT_EINVAL, "grid counts must be >= 1");
+    if ((double)g.nx * g.ny * g.nz > 2147483647.0) return fail(SMCRT_ERANGE, "grid has more than 2^31-1 voxels");
+    if (!(g.xmax > 0 && g.ymax > 0 && g.zmax > 0)) return fail(SMCRT_EINVAL, "grid half sizes must be > 0");
+
+    smcrt_scene* s = new smcrt_scene();
+    s->grid = g;
+    s->src = d->source;
+    s->mats.assign(d->materials, d->materials + d->n_materials);
+    for (int r = 0; r < d->n_regions; ++r) {
+        const smcrt_region& R = d->regions[r];
+        if (R.n_nodes < 1 || R.first_node < 0 || R.first_node + R.n_nodes > d->n_nodes) {
+            delete s;
+            return fail(SMCRT_EINVAL, "region %d node range [%d, %d) outside the program", r, R.first_node,
+                        R.first_node + R.n_nodes);
+        }
+        if (R.material < 0 || R.material >= d->n_materials) {
+            delete s;
+            return fail(SMCRT_EINVAL, "region %d material %d out of range", r, R.material);
+        }
+        int nv = 0, np = 1;
+        DevRegion dr{(int)s->nodes.size(), R.n_nodes, R.material, 0};
+        for (int i = 0; i < R.n_nodes; ++i) {
+            const smcrt_node& n = d->nodes[R.first_node + i];
+            bool ok_kind = false;
+            switch (n.op) {
+            case SMCRT_OP_PRIM: ok_kind = n.kind >= SMCRT_SPHERE && n.kind <= SMCRT_PLANE; break;
+            case SMCRT_OP_CSG: ok_kind = n.kind >= SMCRT_UNION && n.kind <= SMCRT_INTERSECTION; break;
+            case SMCRT_OP_MOD_BEGIN:
+            case SMCRT_OP_MOD_END: ok_kind = n.kind >= SMCRT_EXTRUDE && n.kind <= SMCRT_BEND; break;
+            default: break;
+            }
+            if (!ok_kind) {
+                delete s;
+                return fail(SMCRT_EINVAL, "region %d node %d: bad op/kind (%d, %d)", r, i, n.op, n.kind);
+            }
+            if (!finite_all(n.p, 8) || !finite_all(n.t, 12) || !std::isfinite(n.k)) {
+                delete s;
+                return fail(SMCRT_EINVAL, "region %d node %d: non-finite parameter", r, i);
+            }
+            if (n.op == SMCRT_OP_CSG && n.kind == SMCRT_SMOOTH_UNION && !(n.k > 0)) {
+                delete s;
+                return fail(SMCRT_EINVAL, "region %d node %d: SmoothUnion needs k > 0", r, i);
+            }
+            if (n.op == SMCRT_OP_CSG && nv < 2) {
+                delete s;
+                return fail(SMCRT_EINVAL, "region %d node %d: operator needs two distances", r, i);
+            }
+            if (n.op == SMCRT_OP_MOD_END && (np < 2 || nv < 1)) {
+                delete s;
+                return fail(SMCRT_EINVAL, "region %d node %d: unmatched modifier end", r, i);
+            }
+            nv += n_values_pushed(n.op);
+            if (n.op == SMCRT_OP_MOD_BEGIN) ++np;
+            if (n.op == SMCRT_OP_MOD_END) --np;
+            if (nv > SMCRT_MAX_VALUE_STACK || np > SMCRT_MAX_POS_STACK) {
+                delete s;
+                return fail(SMCRT_ERANGE, "region %d: program deeper than the evaluation stacks (%d/%d)", r,
+                            SMCRT_MAX_VALUE_STACK, SMCRT_MAX_POS_STACK);
+            }
+            DevNode dn;
+            memset(&dn, 0, sizeof dn);
+            dn.op = n.op;
+            dn.kind = n.kind;
+            dn.k = (float)n.k;
+            for (int j = 0; j < 8; ++j) dn.p[j] = (float)n.p[j];
+            bool ident = true;
+            static const double I12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+            for (int j = 0; j < 12; ++j) {
+                dn.t[j] = (float)n.t[j];
+                if (n.t[j] != I12[j]) ident = false;
+            }
+            dn.ident = ident ? 1 : 0;
+            s->nodes.push_back(dn);
+        }
+        if (nv != 1 || np != 1) {
+            delete s;
+            return fail(SMCRT_EINVAL, "region %d: program leaves %d distances and %d positions (want 1, 1)", r, nv, np);
+        }
+        s->regions.push_back(dr);
+        s->region_mat.push_back(R.material);
+    }
+
+    // source
+    const smcrt_source& src = d->source;
+    if (src.kind < SMCRT_SRC_PENCIL || src.kind > SMCRT_SRC_POINT) {
+        delete s;
+        return fail(SMCRT_EINVAL, "No such source! (kind %d)", src.kind);
+    }
+    double dn = std::sqrt(src.dir[0] * src.dir[0] + src.dir[1] * src.dir[1] + src.dir[2] * src.dir[2]);
+    if (src.kind != SMCRT_SRC_POINT && std::fabs(dn - 1.0) > 1e-6) {
+        delete s;
+        return fail(SMCRT_EINVAL, "source direction must be a unit vector (|dir| = %g)", dn);
+    }
+    if (src.kind == SMCRT_SRC_CIRCULAR && !(src.radius >= 0)) {
+        delete s;
+        return fail(SMCRT_EINVAL, "circular source radius must be >= 0");
+    }
+    // circular: t = rotationAlign(a, dir) . translate(-position) (src/photon.f90:238-266)
+    {
+        double b[3] = {src.dir[0], src.dir[1], src.dir[2]};
+        if (dn > 0) for (double& v : b) v /= dn;
+        double a[3] = {1, 0, 0};
+        s->circ_swap = 0;
+        if (std::fabs(b[0]) == 1.0 && b[1] == 0.0 && b[2] == 0.0) {
+            a[0] = 0; a[2] = 1;
+            s->circ_swap = 1;
+        }
+        double v[3] = {a[1] * b[2] - a[2] * b[1], -a[0] * b[2] + a[2] * b[0], a[0] * b[1] - a[1] * b[0]};
+        double c = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+        double k = 1.0 / (1.0 + c);
+        double vx[3][3] = {{0, v[2], -v[1]}, {-v[2], 0, v[0]}, {v[1], -v[0], 0}};  // [row][col]
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s2 = 0;
+                for (int m = 0; m < 3; ++m) s2 += vx[i][m] * vx[m][j];
+                s->circ_t[i * 3 + j] = (float)((i == j ? 1.0 : 0.0) + vx[i][j] + s2 * k);
+            }
+        for (int j = 0; j < 3; ++j) s->circ_t[9 + j] = (float)(-src.pos[j]);
+    }
+    if (src.kind == SMCRT_SRC_POINT) { s->rdir[0] = 0; s->rdir[1] = 0; s->rdir[2] = -1; }
+    else for (int j = 0; j < 3; ++j) s->rdir[j] = (float)(src.dir[j] / dn);
+
+    // epsilons: the march/crossing epsilon is scaled to FP32 resolution at the scene's size
+    // (2^-17 of the largest grid half-size, about 64 ulp); the kill threshold after emission or
+    // a scatter keeps the reference's 1e-8 (src/inttau2.f90:44)
+    double scale = std::fmax(g.xmax, std::fmax(g.ymax, g.zmax));
+    s->eps = (float)std::fmax(1e-8, scale * (1.0 / 131072.0));
+    s->eps_kill = 1e-8f;
+    *out = s;
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
+    if (!s) return SMCRT_OK;
+    if (s->device >= 0) {
+        (void)hipSetDevice(s->device);
+        DevBuf* bufs[] = {&s->d_nodes, &s->d_regions, &s->d_mats, &s->d_region_mat, &s->d_mat_mus, &s->d_work,
+                          &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights,
+                          &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum};
+        for (DevBuf* b : bufs) b->release();
+        if (s->ev0) (void)hipEventDestroy(s->ev0);
+        if (s->ev1) (void)hipEventDestroy(s->ev1);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+    }
+    delete s;
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_scene_set_materials(smcrt_scene* s, const smcrt_material* m, int32_t n) {
+    if (!s || !m) return fail(SMCRT_EINVAL, "null scene or materials");
+    if (n != (int32_t)s->mats.size()) return fail(SMCRT_EINVAL, "expected %zu materials, got %d", s->mats.size(), n);
+    int rc = validate_materials(m, n);
+    if (rc) return rc;
+    s->mats.assign(m, m + n);
+    s->mats_dirty = true;
+    return SMCRT_OK;
+}
+
+static int ensure_device(smcrt_scene* s) {
+    if (s->device < 0) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SMCRT_EDEVICE, "no HIP device available");
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        s->device = dev;
+        HIPCHECK(hipDeviceGetAttribute(&s->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHECK(hipStreamCreate(&s->stream));
+        HIPCHECK(hipEventCreate(&s->ev0));
+        HIPCHECK(hipEventCreate(&s->ev1));
+    }
+    HIPCHECK(hipSetDevice(s->device));
+    if (!s->uploaded) {
+        int rc;
+        if ((rc = s->d_nodes.ensure(s->nodes.size() * sizeof(DevNode)))) return rc;
+        if ((rc = s->d_regions.ensure(s->regions.size() * sizeof(DevRegion)))) return rc;
+        if ((rc = s->d_region_mat.ensure(s->region_mat.size() * sizeof(int)))) return rc;
+        if ((rc = s->d_work.ensure(8 * sizeof(unsigned long long)))) return rc;
+        if ((rc = s->d_stats.ensure((NOUT + NEV) * sizeof(unsigned long long)))) return rc;
+        if ((rc = s->d_moments.ensure(24 * sizeof(double)))) return rc;
+        if ((rc = s->d_setinfo.ensure(4 * sizeof(unsigned long long)))) return rc;
+        if ((rc = s->d_sum.ensure(sizeof(double)))) return rc;
+        HIPCHECK(hipMemcpy(s->d_nodes.p, s->nodes.data(), s->nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(s->d_regions.p, s->regions.data(), s->regions.size() * sizeof(DevRegion), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(s->d_region_mat.p, s->region_mat.data(), s->region_mat.size() * sizeof(int), hipMemcpyHostToDevice));
+        s->uploaded = true;
+    }
+    if (s->mats_dirty) {
+        int rc;
+        std::vector<DevMat> dm(s->regions.size());
+        region_mats(s, s->mats.data(), dm.data());
+        std::vector<float> mus(s->mats.size());
+        for (size_t i = 0; i < s->mats.size(); ++i) mus[i] = (float)s->mats[i].mus;
+        if ((rc = s->d_mats.ensure(dm.size() * sizeof(DevMat)))) return rc;
+        if ((rc = s->d_mat_mus.ensure(mus.size() * sizeof(float)))) return rc;
+        HIPCHECK(hipMemcpy(s->d_mats.p, dm.data(), dm.size() * sizeof(DevMat), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(s->d_mat_mus.p, mus.data(), mus.size() * sizeof(float), hipMemcpyHostToDevice));
+        s->mats_dirty = false;
+    }
+    return SMCRT_OK;
+}
+
+static void fill_common(const smcrt_scene* s, KParams& P) {
+    memset(&P, 0, sizeof P);
+    P.nodes = s->d_nodes.as<DevNode>();
+    P.regions = s->d_regions.as<DevRegion>();
+    P.mats = s->d_mats.as<DevMat>();
+    P.nreg = (int)s->regions.size();
+    P.nmat = (int)s->mats.size();
+    P.nx = s->grid.nx; P.ny = s->grid.ny; P.nz = s->grid.nz;
+    P.xmax = (float)s->grid.xmax; P.ymax = (float)s->grid.ymax; P.zmax = (float)s->grid.zmax;
+    P.sx = (float)(s->grid.nx / (2.0 * s->grid.xmax));
+    P.sy = (float)(s->grid.ny / (2.0 * s->grid.ymax));
+    P.sz = (float)(s->grid.nz / (2.0 * s->grid.zmax));
+    P.src_kind = s->src.kind;
+    P.circ_swap = s->circ_swap;
+    for (int j = 0; j < 3; ++j) {
+        P.spos[j] = (float)s->src.pos[j];
+        P.sdir[j] = (float)s->src.dir[j];
+        P.p1[j] = (float)s->src.p1[j];
+        P.p2[j] = (float)s->src.p2[j];
+        P.p3[j] = (float)s->src.p3[j];
+        P.rdir[j] = s->rdir[j];
+    }
+    P.radius = (float)s->src.radius;
+    memcpy(P.circ_t, s->circ_t, sizeof P.circ_t);
+    P.eps = s->eps;
+    P.eps_kill = s->eps_kill;
+    P.region_mat = s->d_region_mat.as<int>();
+    P.mat_mus = s->d_mat_mus.as<float>();
+}
+
+struct LaunchShape {
+    int block, blocks, chunk;
+    size_t lds;
+};
+
+static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj) {
+    LaunchShape L;
+    L.block = (o && o->block_size > 0) ? o->block_size : 256;
+    int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
+    L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
+    L.lds = 0;
+    if (adj) {
+        const size_t per_lane = (size_t)48 * s->mats.size();
+        while (L.block > 64 && per_lane * L.block > 65536) L.block /= 2;
+        L.lds = per_lane * L.block;
+    }
+    L.block = (L.block / 64) * 64;
+    if (L.block < 64) L.block = 64;
+    if (L.block > 256) L.block = 256;
+    if (!(o && o->blocks_per_cu > 0)) {
+        // persistent grid: as many blocks as can be resident at once
+        int active = 0;
+        hipError_t e = adj ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<true>, L.block, L.lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<false>, L.block, 0);
+        if (e == hipSuccess && active > 0) bpc = active;
+    }
+    L.blocks = s->num_cu * bpc;
+    return L;
+}
+
+// Launch the packet kernel for one scene (one set).  d_jmean may be null (no tally).
+static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
+                      hipStream_t stream, float* kernel_ms) {
+    if (o->n_packets == 0) {
+        *kernel_ms = 0.f;
+        return SMCRT_OK;
+    }
+    if (o->trace_packets < 0 || o->trace_events < 0) return fail(SMCRT_EINVAL, "negative trace sizes");
+    KParams P;
+    fill_common(s, P);
+    P.jmean = d_jmean;
+    P.grid_stride = 0;
+    P.seed_lo = (uint32_t)o->seed;
+    P.seed_hi = (uint32_t)(o->seed >> 32);
+    P.total = o->n_packets;
+    P.nsets = 1;
+    unsigned long long info[4] = {0ull, o->n_packets, o->first_packet, 0ull};
+    HIPCHECK(hipMemcpyAsync(s->d_setinfo.p, info, sizeof info, hipMemcpyHostToDevice, stream));
+    P.set_prefix = s->d_setinfo.as<unsigned long long>();
+    P.set_first = s->d_setinfo.as<unsigned long long>() + 2;
+    P.work = s->d_work.as<unsigned long long>();
+    P.stats = s->d_stats.as<unsigned long long>();
+    P.moments = s->d_moments.as<double>();
+    if (o->trace_packets > 0 && o->trace_events > 0) {
+        size_t tb = (size_t)o->trace_packets * o->trace_events * 4 * sizeof(float);
+        int rc = s->d_trace.ensure(tb);
+        if (rc) return rc;
+        HIPCHECK(hipMemsetAsync(s->d_trace.p, 0, tb, stream));
+        P.trace = s->d_trace.as<float>();
+        P.trace_packets = o->trace_packets;
+        P.trace_events = o->trace_events;
+    }
+    P.flags = o->flags;
+    P.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
+    LaunchShape sh = pick_shape(s, o, adj);
+    P.chunk = sh.chunk;
+    P.weights = d_w;
+    P.grad = s->d_grad.as<double>();
+    k_fill_u64<<<1, 64, 0, stream>>>(P.work, 8);
+    k_fill_u64<<<1, 64, 0, stream>>>(P.stats, NOUT + NEV);
+    k_fill_f64<<<1, 64, 0, stream>>>(P.moments, 24);
+    if (adj) k_fill_f64<<<1, 256, 0, stream>>>(P.grad, 4 * s->mats.size() + 1);
+    HIPCHECK(hipEventRecord(s->ev0, stream));
+    if (adj) hipLaunchKernelGGL(packet_kernel<true>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
+    else hipLaunchKernelGGL(packet_kernel<false>, dim3(sh.blocks), dim3(sh.block), 0, stream, P);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(s->ev1, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
+    return SMCRT_OK;
+}
+
+static int read_stats(smcrt_scene* s, smcrt_stats* st, float kernel_ms) {
+    unsigned long long h[NOUT + NEV];
+    double mom[24];
+    HIPCHECK(hipMemcpy(h, s->d_stats.p, sizeof h, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(mom, s->d_moments.p, sizeof mom, hipMemcpyDeviceToHost));
+    memset(st, 0, sizeof *st);
+    st->launched = h[O_LAUNCHED];
+    st->absorbed = h[O_ABSORBED];
+    st->escaped_r = h[O_R];
+    st->escaped_t = h[O_T];
+    st->killed_surface = h[O_KSURF];
+    st->killed_bounce = h[O_KBOUNCE];
+    st->scatters = h[O_SCATTERS];
+    const unsigned long long* e = h + NOUT;
+    st->ev_sweeps = e[E_SWEEPS];
+    st->ev_region_evals = e[E_REGION];
+    st->ev_normals = e[E_NORMALS];
+    st->ev_steps = e[E_STEPS];
+    st->ev_crossings = e[E_CROSS];
+    st->ev_interfaces = e[E_IFACE];
+    st->ev_fresnel = e[E_FRESNEL];
+    st->err_unreachable = e[E_UNREACH];
+    st->err_nan = e[E_NAN];
+    memcpy(st->moments, mom, sizeof mom);
+    st->kernel_ms = kernel_ms;
+    return SMCRT_OK;
+}
+
+static int device_sum(smcrt_scene* s, const double* d, size_t n, hipStream_t stream, double* out) {
+    k_fill_f64<<<1, 64, 0, stream>>>(s->d_sum.as<double>(), 1);
+    k_sum_f64<<<(unsigned)fill_blocks(n), 256, 0, stream>>>(d, n, s->d_sum.as<double>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, s->d_sum.p, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_run_device(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, smcrt_stats* st,
+                                void* stream) {
+    if (!s || !o) return fail(SMCRT_EINVAL, "null scene or options");
+    int rc = ensure_device(s);
+    if (rc) return rc;
+    hipStream_t strm = stream ? (hipStream_t)stream : s->stream;
+    float ms = 0.f;
+    if ((o->flags & SMCRT_FLAG_NO_TALLY)) d_jmean = nullptr;
+    if ((rc = launch_run(s, o, d_jmean, false, nullptr, strm, &ms))) return rc;
+    if (st) {
+        if ((rc = read_stats(s, st, ms))) return rc;
+        if (d_jmean) {
+            size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+            if ((rc = device_sum(s, d_jmean, nv, strm, &st->tally_sum))) return rc;
+        }
+    }
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_run(smcrt_scene* s, const smcrt_run_opts* o, smcrt_tallies* out) {
+    if (!s || !o || !out) return fail(SMCRT_EINVAL, "null scene, options or output");
+    int rc = ensure_device(s);
+    if (rc) return rc;
+    const size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+    const bool tally = !(o->flags & SMCRT_FLAG_NO_TALLY) && (out->jmean || out->jmean_f32);
+    double* dj = nullptr;
+    if (tally) {
+        if ((rc = s->d_jmean.ensure(nv * sizeof(double)))) return rc;
+        dj = s->d_jmean.as<double>();
+        k_fill_f64<<<(unsigned)fill_blocks(nv), 256, 0, s->stream>>>(dj, nv);
+        HIPCHECK(hipGetLastError());
+    }
+    float ms = 0.f;
+    if ((rc = launch_run(s, o, dj, false, nullptr, s->stream, &ms))) return rc;
+    if ((rc = read_stats(s, &out->stats, ms))) return rc;
+    if (tally) {
+        std::vector<double> tmp;
+        double* host = out->jmean;
+        if (!host) {
+            tmp.resize(nv);
+            host = tmp.data();
+        }
+        HIPCHECK(hipMemcpy(host, dj, nv * sizeof(double), hipMemcpyDeviceToHost));
+        double sum = 0.0;
+        for (size_t i = 0; i < nv; ++i) sum += host[i];
+        out->stats.tally_sum = sum;
+        if (out->jmean_f32)
+            for (size_t i = 0; i < nv; ++i) out->jmean_f32[i] = (float)host[i];
+    }
+    if (out->trace && o->trace_packets > 0 && o->trace_events > 0) {
+        HIPCHECK(hipMemcpy(out->trace, s->d_trace.p, (size_t)o->trace_packets * o->trace_events * 4 * sizeof(float),
+                           hipMemcpyDeviceToHost));
+    }
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_evaluate(smcrt_scene* s, int32_t region, int64_t n, const float* pts, float* dist,
+                              float* normals) {
+    if (!s || !pts || n < 0) return fail(SMCRT_EINVAL, "bad arguments");
+    if (region < 0 || region >= (int32_t)s->regions.size()) return fail(SMCRT_EINVAL, "region %d out of range", region);
+    if (n == 0) return SMCRT_OK;
+    int rc = ensure_device(s);
+    if (rc) return rc;
+    if ((rc = s->d_pts.ensure((size_t)n * 3 * sizeof(float)))) return rc;
+    if ((rc = s->d_out.ensure((size_t)n * 4 * sizeof(float)))) return rc;
+    HIPCHECK(hipMemcpy(s->d_pts.p, pts, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+    float* dd = dist ? s->d_out.as<float>() : nullptr;
+    float* dn = normals ? s->d_out.as<float>() + n : nullptr;
+    const DevRegion reg = s->regions[region];
+    hipLaunchKernelGGL(k_evaluate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, s->d_nodes.as<DevNode>(),
+                       reg, (long long)n, s->d_pts.as<float>(), dd, dn);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (dist) HIPCHECK(hipMemcpy(dist, dd, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    if (normals) HIPCHECK(hipMemcpy(normals, dn, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_grad(smcrt_scene* s, const smcrt_run_opts* o, const float* weights, double* jmean,
+                          smcrt_grad_out* out) {
+    if (!s || !o || !out) return fail(SMCRT_EINVAL, "null scene, options or output");
+    if (s->mats.size() > 16) return fail(SMCRT_ERANGE, "the adjoint supports at most 16 materials (scene has %zu)",
+                                         s->mats.size());
+    int rc = ensure_device(s);
+    if (rc) return rc;
+    const size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+    const size_t nm = s->mats.size();
+    if ((rc = s->d_grad.ensure((4 * nm + 1) * sizeof(double)))) return rc;
+    const float* dw = nullptr;
+    if (weights) {
+        if ((rc = s->d_weights.ensure(nv * sizeof(float)))) return rc;
+        HIPCHECK(hipMemcpy(s->d_weights.p, weights, nv * sizeof(float), hipMemcpyHostToDevice));
+        dw = s->d_weights.as<float>();
+    }
+    double* dj = nullptr;
+    if (jmean) {
+        if ((rc = s->d_jmean.ensure(nv * sizeof(double)))) return rc;
+        dj = s->d_jmean.as<double>();
+        k_fill_f64<<<(unsigned)fill_blocks(nv), 256, 0, s->stream>>>(dj, nv);
+    }
+    float ms = 0.f;
+    if ((rc = launch_run(s, o, dj, true, dw, s->stream, &ms))) return rc;
+    if ((rc = read_stats(s, &out->stats, ms))) return rc;
+    std::vector<double> g(4 * nm + 1);
+    HIPCHECK(hipMemcpy(g.data(), s->d_grad.p, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double N = (double)(o->n_packets ? o->n_packets : 1);
+    double* dst[4] = {out->d_r_dmua, out->d_r_dmus, out->d_w_dmua, out->d_w_dmus};
+    for (int w = 0; w < 4; ++w)
+        if (dst[w])
+            for (size_t m = 0; m < nm; ++m) dst[w][m] = g[w * nm + m] / N;
+    out->w_tally = g[4 * nm] / N;
+    out->r_d = (double)out->stats.escaped_r / N;
+    if (jmean) {
+        HIPCHECK(hipMemcpy(jmean, dj, nv * sizeof(double), hipMemcpyDeviceToHost));
+        double sum = 0.0;
+        for (size_t i = 0; i < nv; ++i) sum += jmean[i];
+        out->stats.tally_sum = sum;
+    }
+    return SMCRT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// batched launch
+// ------------------------------------------------------------------------------------------
+struct BatchGroup {
+    smcrt_scene* scene;
+    std::vector<int> sets;  // batch set indices in this group
+    KParams P;
+    LaunchShape sh;
+    size_t grid_off;        // first double of this group's grids
+    size_t stats_off;       // first counter of this group's stats
+};
+
+struct smcrt_batch {
+    std::vector<smcrt_scene*> scenes;
+    std::vector<BatchGroup> groups;
+    std::vector<int> set_group, set_slot;
+    std::vector<size_t> set_grid_off;
+    std::vector<size_t> set_nv;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf d_mats, d_grids, d_stats, d_info, d_work;
+    size_t total_doubles = 0, total_stats = 0;
+};
+
+extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt_batch_set* sets,
+                                  int32_t n_sets, uint64_t seed, smcrt_batch** out) {
+    if (!scenes || !sets || !out || n_scenes < 1 || n_sets < 1) return fail(SMCRT_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < n_scenes; ++i) {
+        if (!scenes[i]) return fail(SMCRT_EINVAL, "scene %d is null", i);
+        int rc = ensure_device(scenes[i]);
+        if (rc) return rc;
+    }
+    smcrt_batch* b = new smcrt_batch();
+    b->scenes.assign(scenes, scenes + n_scenes);
+    b->device = scenes[0]->device;
+    for (int i = 0; i < n_sets; ++i) {
+        const smcrt_batch_set& st = sets[i];
+        if (st.scene < 0 || st.scene >= n_scenes) {
+            delete b;
+            return fail(SMCRT_EINVAL, "set %d: scene %d out of range", i, st.scene);
+        }
+        if (!st.materials) {
+            delete b;
+            return fail(SMCRT_EINVAL, "set %d: null material table", i);
+        }
+        int rc = validate_materials(st.materials, (int)scenes[st.scene]->mats.size());
+        if (rc) { delete b; return rc; }
+    }
+    // groups: one per scene with sets
+    b->set_group.assign(n_sets, -1);
+    b->set_slot.assign(n_sets, -1);
+    b->set_grid_off.assign(n_sets, 0);
+    b->set_nv.assign(n_sets, 0);
+    for (int sc = 0; sc < n_scenes; ++sc) {
+        BatchGroup g;
+        g.scene = scenes[sc];
+        for (int i = 0; i < n_sets; ++i)
+            if (sets[i].scene == sc) g.sets.push_back(i);
+        if (!g.sets.empty()) b->groups.push_back(g);
+    }
+    // sizes
+    size_t nmats = 0, ndoubles = 0, nstats = 0, ninfo = 0;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        BatchGroup& g = b->groups[gi];
+        const smcrt_scene* s = g.scene;
+        const size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+        g.grid_off = ndoubles;
+        g.stats_off = nstats;
+        for (size_t k = 0; k < g.sets.size(); ++k) {
+            int i = g.sets[k];
+            b->set_group[i] = (int)gi;
+            b->set_slot[i] = (int)k;
+            b->set_grid_off[i] = ndoubles;
+            b->set_nv[i] = nv;
+            ndoubles += nv;
+        }
+        nmats += g.sets.size() * s->regions.size();
+        nstats += g.sets.size() * NOUT + NEV;
+        ninfo += 2 * g.sets.size() + 1;
+    }
+    b->total_doubles = ndoubles;
+    b->total_stats = nstats;
+    HIPCHECK(hipSetDevice(b->device));
+    int rc;
+    if ((rc = b->d_mats.ensure(nmats * sizeof(DevMat))) || (rc = b->d_grids.ensure(ndoubles * sizeof(double))) ||
+        (rc = b->d_stats.ensure(nstats * sizeof(unsigned long long))) ||
+        (rc = b->d_info.ensure(ninfo * sizeof(unsigned long long))) ||
+        (rc = b->d_work.ensure(b->groups.size() * sizeof(unsigned long long)))) {
+        smcrt_batch_destroy(b);
+        return rc;
+    }
+    std::vector<DevMat> hm(nmats);
+    std::vector<unsigned long long> info(ninfo);
+    size_t mo = 0, io = 0;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        BatchGroup& g = b->groups[gi];
+        smcrt_scene* s = g.scene;
+        const size_t nr = s->regions.size();
+        const size_t mat_off = mo, info_off = io;
+        unsigned long long acc = 0;
+        for (size_t k = 0; k < g.sets.size(); ++k) {
+            const smcrt_batch_set& st = sets[g.sets[k]];
+            region_mats(s, st.materials, hm.data() + mo);
+            mo += nr;
+            info[io + k] = acc;
+            info[io + g.sets.size() + 1 + k] = st.first_packet;
+            acc += st.n_packets;
+        }
+        info[io + g.sets.size()] = acc;
+        io += 2 * g.sets.size() + 1;
+        KParams& P = g.P;
+        fill_common(s, P);
+        P.mats = b->d_mats.as<DevMat>() + mat_off;
+        P.jmean = b->d_grids.as<double>() + g.grid_off;
+        P.grid_stride = (unsigned long long)s->grid.nx * s->grid.ny * s->grid.nz;
+        P.seed_lo = (uint32_t)seed;
+        P.seed_hi = (uint32_t)(seed >> 32);
+        P.total = acc;
+        P.nsets = (int)g.sets.size();
+        P.set_prefix = b->d_info.as<unsigned long long>() + info_off;
+        P.set_first = b->d_info.as<unsigned long long>() + info_off + g.sets.size() + 1;
+        P.work = b->d_work.as<unsigned long long>() + gi;
+        P.stats = b->d_stats.as<unsigned long long>() + g.stats_off;
+        P.moments = s->d_moments.as<double>();
+        P.max_bounces = 1000;
+        g.sh = pick_shape(s, nullptr, false);
+        P.chunk = g.sh.chunk;
+    }
+    HIPCHECK(hipMemcpy(b->d_mats.p, hm.data(), nmats * sizeof(DevMat), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(b->d_info.p, info.data(), ninfo * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    HIPCHECK(hipStreamCreate(&b->stream));
+    // capture: clear grids, counters and work queues with fill kernels (no memset nodes), then
+    // one packet-kernel node per geometry group
+    HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    k_fill_f64<<<(unsigned)fill_blocks(ndoubles), 256, 0, b->stream>>>(b->d_grids.as<double>(), ndoubles);
+    k_fill_u64<<<(unsigned)fill_blocks(nstats), 256, 0, b->stream>>>(b->d_stats.as<unsigned long long>(), nstats);
+    k_fill_u64<<<1, 64, 0, b->stream>>>(b->d_work.as<unsigned long long>(), b->groups.size());
+    for (BatchGroup& g : b->groups)
+        hipLaunchKernelGGL(packet_kernel<false>, dim3(g.sh.blocks), dim3(g.sh.block), 0, b->stream, g.P);
+    hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
+    if (ce != hipSuccess) {
+        smcrt_batch_destroy(b);
+        return fail(SMCRT_EDEVICE, "graph capture failed: %s", hipGetErrorString(ce));
+    }
+    HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    *out = b;
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_batch_run(smcrt_batch* b) {
+    if (!b) return fail(SMCRT_EINVAL, "null batch");
+    HIPCHECK(hipSetDevice(b->device));
+    HIPCHECK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_batch_read(smcrt_batch* b, int32_t set, double* jmean, smcrt_stats* st) {
+    if (!b) return fail(SMCRT_EINVAL, "null batch");
+    if (set < 0 || set >= (int32_t)b->set_group.size()) return fail(SMCRT_EINVAL, "set %d out of range", set);
+    HIPCHECK(hipSetDevice(b->device));
+    const BatchGroup& g = b->groups[b->set_group[set]];
+    if (jmean)
+        HIPCHECK(hipMemcpy(jmean, b->d_grids.as<double>() + b->set_grid_off[set], b->set_nv[set] * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    if (st) {
+        unsigned long long h[NOUT];
+        HIPCHECK(hipMemcpy(h, b->d_stats.as<unsigned long long>() + g.stats_off + (size_t)b->set_slot[set] * NOUT,
+                           sizeof h, hipMemcpyDeviceToHost));
+        memset(st, 0, sizeof *st);
+        st->launched = h[O_LAUNCHED];
+        st->absorbed = h[O_ABSORBED];
+        st->escaped_r = h[O_R];
+        st->escaped_t = h[O_T];
+        st->killed_surface = h[O_KSURF];
+        st->killed_bounce = h[O_KBOUNCE];
+        st->scatters = h[O_SCATTERS];
+        if (jmean) {
+            double sum = 0.0;
+            for (size_t i = 0; i < b->set_nv[set]; ++i) sum += jmean[i];
+            st->tally_sum = sum;
+        }
+    }
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_batch_destroy(smcrt_batch* b) {
+    if (!b) return SMCRT_OK;
+    if (b->device >= 0) (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->d_mats.release();
+    b->d_grids.release();
+    b->d_stats.release();
+    b->d_info.release();
+    b->d_work.release();
+    delete b;
+    return SMCRT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// output helpers (src/writer.f90)
+// ------------------------------------------------------------------------------------------
+extern "C" int smcrt_normalise_fluence(const smcrt_grid* g, const double* raw, uint64_t n, float* out) {
+    if (!g || !raw || !out || n == 0) return fail(SMCRT_EINVAL, "bad arguments");
+    const double dx = 2.0 * g->xmax / g->nx, dy = 2.0 * g->ymax / g->ny, dz = 2.0 * g->zmax / g->nz;
+    const double f = (2.0 * g->xmax * 2.0 * g->ymax) / ((double)n * dx * dy * dz);
+    const size_t nv = (size_t)g->nx * g->ny * g->nz;
+    for (size_t i = 0; i < nv; ++i) out[i] = (float)(raw[i] * f);
+    return SMCRT_OK;
+}
+
+extern "C" int smcrt_write_nrrd(const char* path, const float* data, int32_t nx, int32_t ny, int32_t nz,
+                                const char* meta) {
+    if (!path || !data || nx < 1 || ny < 1 || nz < 1) return fail(SMCRT_EINVAL, "bad arguments");
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(SMCRT_EIO, "cannot open %s for writing", path);
+    fprintf(f, "NRRD0004\ntype: float\ndimension: 3\nsizes: %d %d %d\nspace dimension: 3\nencoding: raw\nendian: little\n",
+            nx, ny, nz);
+    if (meta && *meta) {
+        fputs(meta, f);
+        size_t l = strlen(meta);
+        if (meta[l - 1] != '\n') fputc('\n', f);
+    }
+    fputs("\n\n", f);  // `write(u,"(A)") new_line("C")`: a newline plus the record end
+    size_t nv = (size_t)nx * ny * nz;
+    size_t w = fwrite(data, sizeof(float), nv, f);
+    int bad = (w != nv) | (fclose(f) != 0);
+    if (bad) return fail(SMCRT_EIO, "short write to %s", path);
+    return SMCRT_OK;
+}

@@ -1,0 +1,248 @@
+"""GPU parity: the HIP packet engine against the FP64 CPU oracle (oracle/liboracle.so).
+
+Bars (stated per test):
+  - SDF distances within 1e-5*(1+|d|) absolute; normals (exact FP32 gradients) within 1e-5 of the
+    oracle's FP64 tetrahedral finite differences away from kinks;
+  - emission positions and the first scatter sites of a packet are the same random stream on both
+    sides (common random numbers), equal to FP32 precision;
+  - whole-loop outputs (absorbed / reflected / transmitted / killed fractions, per-cell fluence)
+    within 3 sigma of the oracle's run-to-run statistical error;
+  - the reference's own end-to-end KATs (test/end_to_end/test_scat.f90) on the GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from signedmcrt_amd import scenes
+from signedmcrt_amd.engine import Engine
+from signedmcrt_amd.scene import Scene, cart_grid, source
+from signedmcrt_amd.sdf import (SmoothUnion, bend, box, capsule, cone, cylinder, egg, elongate,
+                                extrude, intersection, invert, model, mono, onion, plane,
+                                revolution, rotate_y, segment, sphere, subtraction, torus,
+                                translate, triprism, twist, union)
+from stats_util import batch_mean_se, coarsen, field_z, two_sample_z
+
+pytestmark = pytest.mark.gpu
+
+M = mono(1.0, 0.1, 0.0, 1.0)
+
+
+def sdf_zoo():
+    """One region per primitive / operator / modifier (src/sdfs/*.f90)."""
+    return [
+        sphere(0.7, M, 1, transform=invert(translate((0.1, -0.2, 0.3)))),
+        box((1.0, 1.4, 0.6), M, 1, transform=rotate_y(30.0)),
+        torus(0.8, 0.25, M, 1),
+        cylinder((0.0, 0.0, -0.5), (0.2, 0.1, 0.6), 0.4, M, 1),
+        triprism(0.8, 0.5, M, 1),
+        segment((-0.5, 0.0, 0.0), (0.6, 0.3, 0.0), M, 1),
+        capsule((-0.4, 0.1, 0.0), (0.5, -0.2, 0.3), 0.3, M, 1),
+        cone((0.0, 0.0, -0.4), (0.0, 0.0, 0.6), 0.6, 0.2, M, 1),
+        egg(0.8, 0.3, 0.5, M, 1),
+        plane((0.0, 0.6, 0.8), M, 1),
+        model([sphere(0.5, M, 1), box((0.8, 0.8, 0.8), M, 1, transform=invert(translate((0.4, 0, 0))))], union),
+        model([sphere(0.5, M, 1), box((0.8, 0.8, 0.8), M, 1, transform=invert(translate((0.4, 0, 0))))], SmoothUnion, 0.2),
+        model([sphere(0.5, M, 1), box((1.2, 1.2, 1.2), M, 1)], subtraction),
+        model([sphere(0.7, M, 1), box((1.0, 1.0, 1.0), M, 1)], intersection),
+        extrude(segment((-0.5, 0.0, 0.0), (0.5, 0.2, 0.0), M, 1), 0.3),
+        revolution(box((0.3, 0.4, 0.3), M, 1), 0.6),
+        onion(sphere(0.6, M, 1), 0.05),
+        elongate(sphere(0.3, M, 1), (0.3, 0.1, 0.2)),
+        twist(box((0.6, 0.3, 1.2), M, 1), 1.5),
+        bend(box((1.2, 0.3, 0.3), M, 1), 0.8),
+    ]
+
+
+def zoo_scene():
+    return Scene(sdf_zoo(), cart_grid(10, 10, 10, 2.0, 2.0, 2.0), source("point"))
+
+
+def test_sdf_values_match_oracle(gpu):
+    sc = zoo_scene()
+    eng = Engine(sc)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(-1.5, 1.5, (4000, 3))
+    for r in range(sc.n_regions):
+        dg = eng.evaluate(r, pts)
+        do = O.evaluate(sc, r, pts)
+        err = np.abs(dg.astype(np.float64) - do)
+        tol = 1e-5 * (1.0 + np.abs(do))
+        assert np.all(err <= tol), (r, type(sc.array[r]).__name__, err.max(), pts[err.argmax()])
+
+
+def test_reference_sdf_vectors_on_gpu(gpu):
+    """test/SDF/test_SDF.f90:676-1034 axis-point values, through the device evaluator."""
+    cases = [
+        (sphere(1.0, M, 1), [(0, 0, 0), (0, 1, 0), (1, 0, 0), (0, 0, -1)], [-1, 0, 0, 0]),
+        (box((2, 2, 2), M, 1), [(0, 0, 0), (0, 1, 0), (1, 1, 1)], [-1, 0, 0]),
+        (cylinder((0, 0, -1), (0, 0, 1), 1.0, M, 1), [(0, 0, 0), (0, 1, 0), (0, 0, 1)], [-1, 0, 0]),
+        (torus(0.5, 1.0, M, 1), [(0, 0, 0), (1.5, 0, 0)], [-0.5, 0]),
+        (segment((-1, 0, 0), (1, 0, 0), M, 1), [(0, 0, 0), (1, 1.1, 0), (0, 0, 1.1)], [-0.1, 1, 1]),
+        (triprism(1.0, 5.0, M, 1), [(0, 0, 5), (0, 1, 0)], [0, 0]),
+        (capsule((-1, 0, 0), (1, 0, 0), 1.0, M, 1), [(0, 0, 0), (0, 1, 0), (2, 0, 0)], [-1, 0, 0]),
+        (plane((0, 0, 1), M, 1), [(0, 0, 0), (0, 0, -1), (0, 0, 1)], [0, -1, 1]),
+        (cone((0, 0, 0), (0, 0, 1), 5.0, 0.0, M, 1), [(0, 0, 1), (1, 1, 0)], [0, 0]),
+        (egg(2.5, 0.75, 1.5, M, 1), [(0, 0, 0), (2.5, 0, 0), (2.5, 2.5, 0)], [-2.5, 0, 0.630294]),
+    ]
+    arr = [c[0] for c in cases]
+    sc = Scene(arr, cart_grid(4, 4, 4, 5.0, 5.0, 5.0), source("point"))
+    eng = Engine(sc)
+    for r, (_, pts, want) in enumerate(cases):
+        got = eng.evaluate(r, np.array(pts, dtype=float))
+        np.testing.assert_allclose(got, want, atol=2e-5, err_msg=type(arr[r]).__name__)
+
+
+def test_normals_match_oracle(gpu):
+    sc = zoo_scene()
+    eng = Engine(sc)
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-1.5, 1.5, (3000, 3))
+    for r in range(sc.n_regions):
+        _, ng = eng.evaluate(r, pts, normals=True)
+        _, no = O.evaluate(sc, r, pts, normals=True)
+        err = np.linalg.norm(ng - no, axis=1)
+        bad = np.mean(err > 1e-4)
+        # kinks (box edges, min/max switches, clamp ends) are measure-zero but the FD stencil
+        # straddles them within h; allow a small fraction there
+        assert bad < 0.01, (r, type(sc.array[r]).__name__, bad)
+        assert np.median(err) < 1e-5, (r, np.median(err))
+
+
+def test_unit_sphere_normals_axes(gpu):
+    """test/SDF/test_SDF.f90:126-168: calcNormal on a unit sphere at the axes."""
+    sc = Scene([sphere(1.0, M, 1)], cart_grid(4, 4, 4, 2, 2, 2), source("point"))
+    pts = np.array([(1, 0, 0), (0, 1, 0), (0, 0, 1)], dtype=float)
+    _, n = Engine(sc).evaluate(0, pts, normals=True)
+    np.testing.assert_allclose(n, np.eye(3), atol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["slab", "refracting_sphere", "aptran", "scat_test"])
+def test_emission_common_random_numbers(gpu, name):
+    """First trace event = emission position; same Philox stream on both sides."""
+    sc, _ = scenes.build(name)
+    P = 500
+    gt = Engine(sc).run(P, seed=99, tally=False, trace_packets=P, trace_events=2).trace
+    pos, _ = O.emit(sc, P, seed=99)
+    assert np.all(gt[:, 0, 3] == 1)
+    np.testing.assert_allclose(gt[:, 0, :3], pos, atol=2e-6)
+
+
+def test_scatter_sites_common_random_numbers(gpu):
+    """scat_test2 has no surface within reach: the first four scatter sites of each packet come
+    from identical draws on both sides and agree to FP32 precision."""
+    sc, _ = scenes.build("scat_test2")
+    P, E = 2000, 8
+    gt = Engine(sc).run(P, seed=5, tally=False, trace_packets=P, trace_events=E).trace
+    ot = O.run(sc, P, seed=5, tally=False, trace_packets=P, trace_events=E)["trace"]
+    for k in range(1, 5):  # events 1..4 are scatters 1..4
+        assert np.all(gt[:, k, 3] == 2) and np.all(ot[:, k, 3] == 2)
+        np.testing.assert_allclose(gt[:, k, :3], ot[:, k, :3], atol=1e-5 * k, rtol=1e-4)
+
+
+def test_kat_mean_scatters(gpu):
+    """Scatter_test1 (test/end_to_end/test_scat.f90:23-41): <N_scatt> = 57.5 +- 0.5."""
+    sc, n = scenes.build("scat_test")
+    st = Engine(sc).run(n, seed=123456789, tally=False).stats
+    assert abs(st["scatters"] / st["launched"] - 57.5) <= 0.5
+    ref = O.run(sc, n, seed=321, tally=False)["stats"]
+    # the per-packet scatter count has sd ~ 40: 3 sigma of the difference of two means
+    diff = st["scatters"] / n - ref["scatters"] / n
+    assert abs(diff) < 3 * np.sqrt(2) * 40 / np.sqrt(n)
+
+
+def test_kat_scatter_moments(gpu):
+    """Scatter_test2 (test/end_to_end/test_scat.f90:43-86): moments of scatter sites 1..4 in
+    mean free paths against the reference's table (thresholds 0.1 in x/y, 0.143 in z)."""
+    sc, n = scenes.build("scat_test2")  # 1e7 packets, as the reference's test
+    st = Engine(sc).run(n, seed=123456789, flags=3, tally=False).stats
+    m = np.array(st["moments"]).reshape(2, 4, 3) / n
+    first, second = 10 * m[0], 100 * m[1]
+    want1 = np.array([[0, 0, 1.0], [0, 0, 1.9], [0, 0, 2.71], [0, 0, 3.349]])
+    want2 = np.array([[0, 0, 2.0], [0.1266666666, 0.1266666666, 5.5466666666],
+                      [0.469933, 0.469933, 10.28013], [1.091246, 1.091246, 15.91551]])
+    # expected <z_4> = 3.4905 sits 0.0015 inside the 0.143 bound: allow 3 standard errors
+    sd = np.sqrt(2.0 * np.arange(1, 5))
+    se = (3 * sd / np.sqrt(n))[:, None]
+    for got, want, s in ((first, want1, se), (second, want2, 4 * se * sd[:, None])):
+        assert np.all(np.abs(got[:, :2] - want[:, :2]) <= 0.1 + s), got
+        assert np.all(np.abs(got[:, 2] - want[:, 2]) <= 0.143 + s[:, 0]), got
+    # and against the oracle at 4 sigma of the moment estimates
+    ref = O.run(sc, n // 4, seed=77, flags=3, tally=False)["stats"]
+    mr = np.array(ref["moments"]).reshape(2, 4, 3) / (n // 4)
+    assert np.all(np.abs(10 * mr[0] - first) < 0.02)
+
+
+def _fractions_z(sg, so):
+    out = {}
+    for key in ("absorbed", "escaped_r", "escaped_t", "killed_surface"):
+        out[key] = two_sample_z(sg[key], sg["launched"], so[key], so["launched"])
+    return out
+
+
+def _field_parity(name, n_batch, B, f, seed0=1000):
+    sc, _ = scenes.build(name)
+    eng = Engine(sc)
+    gb, ob, sg, so = [], [], None, None
+    for b in range(B):
+        rg = eng.run(n_batch, seed=seed0, first_packet=b * n_batch)
+        ro = O.run(sc, n_batch, seed=seed0 + 1, first_packet=b * n_batch)
+        gb.append(coarsen(rg.jmean, f) / n_batch)
+        ob.append(coarsen(ro["jmean"], f) / n_batch)
+        sg = rg.stats if sg is None else {k: sg[k] + rg.stats[k] for k in sg if k in rg.stats and not isinstance(sg[k], list)}
+        so = ro["stats"] if so is None else {k: so[k] + ro["stats"][k] for k in so if not isinstance(so[k], list)}
+    mg, seg = batch_mean_se(gb)
+    mo, seo = batch_mean_se(ob)
+    return sg, so, field_z(mg, seg, mo, seo)
+
+
+def _check_balance(st):
+    assert st["absorbed"] + st["escaped_r"] + st["escaped_t"] + st["killed_surface"] + st["killed_bounce"] == st["launched"]
+    assert st["err_unreachable"] == 0 and st["err_nan"] == 0
+
+
+def test_slab_parity(gpu):
+    """Config 1 (slab, pencil beam): fractions and per-cell fluence within 3 sigma."""
+    sg, so, z = _field_parity("slab", 62_500, 16, 5)
+    _check_balance(sg)
+    for key, zz in _fractions_z(sg, so).items():
+        assert abs(zz) < 3.0, (key, zz, sg[key], so[key])
+    assert len(z) > 200
+    assert np.mean(np.abs(z) > 3) < 0.03, np.sort(np.abs(z))[-10:]
+    assert abs(np.mean(z)) < 0.25
+
+
+def test_refracting_sphere_parity(gpu):
+    """Config 2 (refracting sphere in a scattering cube, circular beam)."""
+    sg, so, z = _field_parity("refracting_sphere", 50_000, 16, 10)
+    _check_balance(sg)
+    for key, zz in _fractions_z(sg, so).items():
+        assert abs(zz) < 3.0, (key, zz, sg[key], so[key])
+    assert len(z) > 200
+    assert np.mean(np.abs(z) > 3) < 0.03, np.sort(np.abs(z))[-10:]
+    assert abs(np.mean(z)) < 0.25
+
+
+def test_counts_independent_of_launch_shape(gpu):
+    """Packet j always draws from stream (seed, j): the termination counts do not depend on
+    block size, grid size or chunking."""
+    sc, _ = scenes.build("refracting_sphere")
+    eng = Engine(sc)
+    a = eng.run(100_000, seed=3, tally=False).stats
+    b = eng.run(100_000, seed=3, tally=False, block=64, blocks_per_cu=1, chunk=7).stats
+    for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface", "scatters", "ev_steps"):
+        assert a[k] == b[k], k
+
+
+def test_run_device_matches_run(gpu):
+    import torch
+    sc, _ = scenes.build("slab")
+    eng = Engine(sc)
+    g = sc.grid
+    d = torch.zeros(g.nxg * g.nyg * g.nzg, dtype=torch.float64, device="cuda")
+    st = eng.run_device(50_000, d.data_ptr(), seed=11)
+    r = eng.run(50_000, seed=11)
+    assert st["escaped_r"] == r.stats["escaped_r"]
+    np.testing.assert_allclose(d.cpu().numpy().reshape(g.shape, order="F"), r.jmean, rtol=1e-9, atol=1e-12)
+    assert abs(st["tally_sum"] / r.jmean.sum() - 1) < 1e-9

@@ -355,6 +355,8 @@ typedef struct {
     double C;                        /* weighted in-grid path tally so far */
     double L[SMCRT_MAX_MATERIALS], A[SMCRT_MAX_MATERIALS], B[SMCRT_MAX_MATERIALS];
     double K[SMCRT_MAX_MATERIALS];
+    float* tr; /* trace slot of this packet (or NULL) */
+    int tc, te;
 } orc_packet;
 
 typedef struct {
@@ -474,6 +476,8 @@ static void sweep(const orc_scene* S, orc_thread* T, v3 p, double* ds) {
     T->sweeps++;
 }
 
+static void trace_put(float* tr, int ne, int* cnt, v3 p, int ev);
+
 /* `tauint2` (src/inttau2.f90:15-210) */
 static void tauint2(const orc_scene* S, orc_thread* T, orc_packet* k, int max_bounces) {
     const double eps = 1e-8;
@@ -549,8 +553,10 @@ static void tauint2(const orc_scene* S, orc_thread* T, orc_packet* k, int max_bo
             taurun = 0.0;
             if (!rflag) {
                 k->layer = new_layer;
+                trace_put(k->tr, k->te, &k->tc, k->pos, 3);
             } else {
                 k->pos = oldpos;
+                trace_put(k->tr, k->te, &k->tc, k->pos, 4);
                 k->bounces++;
                 if (k->bounces > max_bounces) { tag_end(k, END_BOUNCE); break; }
             }
@@ -693,8 +699,10 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
             rng_init(&k.rng, o->seed, o->first_packet + (uint64_t)j);
             emit(&S, &k);
             float* tr = (o->trace_packets > j && out->trace) ? out->trace + (size_t)j * o->trace_events * 4 : NULL;
-            int tc = 0;
-            trace_put(tr, o->trace_events, &tc, k.pos, 1);
+            k.tr = tr;
+            k.te = o->trace_events;
+            k.tc = 0;
+            trace_put(tr, o->trace_events, &k.tc, k.pos, 1);
             /* layer at emission (src/kernelsMod.f90:275-280) */
             {
                 double best = INFINITY;
@@ -719,7 +727,7 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
                     scatter(&k, S.hgg[k.layer], S.g2[k.layer]);
                     nsc++;
                     a_sc++;
-                    trace_put(tr, o->trace_events, &tc, k.pos, 2);
+                    trace_put(tr, o->trace_events, &k.tc, k.pos, 2);
                     if ((o->flags & SMCRT_FLAG_MOMENTS) && nsc <= 4) {
                         mom[0][nsc - 1][0] += k.pos.x; mom[0][nsc - 1][1] += k.pos.y; mom[0][nsc - 1][2] += k.pos.z;
                         mom[1][nsc - 1][0] += k.pos.x * k.pos.x; mom[1][nsc - 1][1] += k.pos.y * k.pos.y;
@@ -734,13 +742,13 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
             }
             int isr = 0;
             switch (k.reason) {
-            case END_ABSORB: a_abs++; trace_put(tr, o->trace_events, &tc, k.pos, 5); break;
+            case END_ABSORB: a_abs++; trace_put(tr, o->trace_events, &k.tc, k.pos, 5); break;
             case END_ESCAPE:
                 if (dot(k.dir, S.src_dir) < 0.0) { a_r++; isr = 1; } else a_t++;
-                trace_put(tr, o->trace_events, &tc, k.pos, 6);
+                trace_put(tr, o->trace_events, &k.tc, k.pos, 6);
                 break;
-            case END_SURFACE: a_ks++; trace_put(tr, o->trace_events, &tc, k.pos, 7); break;
-            case END_BOUNCE: a_kb++; trace_put(tr, o->trace_events, &tc, k.pos, 7); break;
+            case END_SURFACE: a_ks++; trace_put(tr, o->trace_events, &k.tc, k.pos, 7); break;
+            case END_BOUNCE: a_kb++; trace_put(tr, o->trace_events, &k.tc, k.pos, 7); break;
             default: a_err++; break;
             }
             if (k.grad) {

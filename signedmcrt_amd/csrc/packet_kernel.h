@@ -434,15 +434,16 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             // src/inttau2.f90:52-63.  After an interface the threshold is the march epsilon (the
             // crossing-step ratio that kills grazing crossings and reflected packets); after
             // emission or a scatter it is the reference's own 1e-8.
-            const float thr = L.iface ? eps : eps_kill;
-            if (dmin < thr) end_kind = END_SURFACE;
-            else if (dmin > eps || !L.iface) do_march = true;
-            else cross = true;  // exactly eps after an interface: straight to the crossing
+            // FP32 distances are quantised (multiples of 2^-24 near |p| = 1), so d == eps ties
+            // that never occur in the reference's FP64 are common here: a tie counts as inside
+            // eps, which kills a reflected packet (pos = oldpos, d0 <= eps) as the reference does.
+            if (L.iface ? (dmin <= eps) : (dmin < eps_kill)) end_kind = END_SURFACE;
+            else do_march = true;
         } else if (st0 == ST_MARCHED) {
             // src/inttau2.f90:85-97
             if (smin >= 0.f) end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
             else if (dmin > eps) do_march = true;
-            else if (dmin < L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
+            else if (dmin <= L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
             else do_march = true;  // within eps but moving away from the surface: keep marching
         } else if (st0 == ST_CROSSED) {
             // the sweep ran at the crossed position (src/inttau2.f90:117-162)

@@ -224,6 +224,35 @@ def test_refracting_sphere_parity(gpu):
     assert abs(np.mean(z)) < 0.25
 
 
+def _outcomes(trace):
+    """Final event kind of each traced packet (5 absorb, 6 escape, 7 kill)."""
+    k = trace[:, :, 3].astype(int)
+    last = np.array([row[row > 0][-1] if np.any(row > 0) else 0 for row in k])
+    return last
+
+
+@pytest.mark.parametrize("name", ["slab", "refracting_sphere"])
+def test_common_random_number_outcomes(gpu, name):
+    """Same packets, same random streams: most histories coincide event by event, so the
+    fraction of packets whose outcome differs is small and its net shift between classes is
+    within counting noise of the differing packets (a bias shows up here long before it shows
+    in independent-sample comparisons)."""
+    sc, _ = scenes.build(name)
+    P, E = 20000, 400
+    gt = Engine(sc).run(P, seed=2024, tally=False, trace_packets=P, trace_events=E).trace
+    ot = O.run(sc, P, seed=2024, tally=False, trace_packets=P, trace_events=E)["trace"]
+    go, oo = _outcomes(gt), _outcomes(ot)
+    full = (gt[:, -1, 3] == 0) & (ot[:, -1, 3] == 0)  # histories that fit the trace
+    go, oo = go[full], oo[full]
+    assert len(go) > 0.9 * P
+    diff = go != oo
+    assert diff.mean() < 0.03, diff.mean()
+    nd = max(int(diff.sum()), 1)
+    for kind in (5, 6, 7):
+        shift = int(np.sum(go == kind)) - int(np.sum(oo == kind))
+        assert abs(shift) <= 3 * np.sqrt(nd) + 2, (kind, shift, nd)
+
+
 def test_counts_independent_of_launch_shape(gpu):
     """Packet j always draws from stream (seed, j): the termination counts do not depend on
     block size, grid size or chunking."""

@@ -121,17 +121,19 @@ def main():
 
     # per-packet fluence error of the GPU estimator: B independent batches
     B = args.error_batches
-    nb = max(n // B, 100_000)
-    gb = []
-    for b in range(B):
-        gg = torch.zeros(nv, dtype=torch.float64, device="cuda")
-        eng.run_device(nb, gg.data_ptr(), seed=99, first_packet=10**9 + b * nb)
-        gb.append((gg.cpu().numpy(), nb))
-    sd_gpu = per_packet_rel_sd(gb)
-    del gb
+    sd_gpu = None
+    if B >= 2:
+        nb = max(n // B, 100_000)
+        gb = []
+        for b in range(B):
+            gg = torch.zeros(nv, dtype=torch.float64, device="cuda")
+            eng.run_device(nb, gg.data_ptr(), seed=99, first_packet=10**9 + b * nb)
+            gb.append((gg.cpu().numpy(), nb))
+        sd_gpu = per_packet_rel_sd(gb)
+        del gb
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and B >= 2:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         threads = len(os.sched_getaffinity(0))

@@ -164,7 +164,8 @@ typedef struct smcrt_run_opts {
     int32_t chunk;         /* packets a wavefront claims per refill, 0 -> default            */
     int32_t trace_packets; /* record the first `trace_packets` packets' events (0 = off)    */
     int32_t trace_events;  /* events per traced packet                                       */
-    int32_t reserved[5];
+    int32_t walk_cap;      /* voxel chords a lane walks per kernel-loop iteration, 0 -> default */
+    int32_t reserved[4];
 } smcrt_run_opts;
 
 /* How a packet ended.  A+R+T+K = launched. */
@@ -185,6 +186,7 @@ typedef struct smcrt_stats {
     uint64_t ev_fresnel;     /* interfaces with n1 != n2                                      */
     uint64_t err_unreachable;/* the reference's "should not be reached" branch (:156)         */
     uint64_t err_nan;        /* non-finite state caught and the packet dropped                */
+    uint64_t ev_atomics;     /* FP64 tally atomics issued (chords merged per voxel)           */
     double moments[2][4][3]; /* SMCRT_FLAG_MOMENTS: sum of pos and pos^2 at scatters 1..4    */
     double kernel_ms;        /* device time of the packet kernel                             */
     double tally_sum;        /* sum of the FP64 fluence accumulator (raw path length)        */

@@ -73,13 +73,13 @@ class RunOpts(C.Structure):
                 ("flags", C.c_uint32), ("max_bounces", C.c_int32), ("block_size", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
                 ("trace_packets", C.c_int32), ("trace_events", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("walk_cap", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 STAT_COUNTERS = ["launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",
                  "killed_bounce", "scatters", "ev_sweeps", "ev_region_evals", "ev_normals",
                  "ev_steps", "ev_crossings", "ev_interfaces", "ev_fresnel", "err_unreachable",
-                 "err_nan"]
+                 "err_nan", "ev_atomics"]
 
 
 class Stats(C.Structure):

@@ -1,8 +1,9 @@
 // packet_kernel.h — the persistent, wavefront-regenerating packet kernel.
 //
-// One lane owns one packet at a time.  Every loop iteration each busy lane performs exactly one
-// SDF sweep (all regions evaluated at its position) and then the state-machine step that the
-// reference runs after that sweep inside `tauint2` (src/inttau2.f90:15-210):
+// One lane owns one packet at a time.  Each loop iteration, every busy lane that is not in the
+// middle of a voxel walk performs one SDF sweep (all regions evaluated at its position) and
+// then the state-machine step that the reference runs after that sweep inside `tauint2`
+// (src/inttau2.f90:15-210):
 //
 //   EMIT     sweep at the emission point: layer = nearest containing surface
 //            (src/kernelsMod.f90:275-280), draw tau, then as START
@@ -13,13 +14,16 @@
 //            Fresnel reflect/refract at the SDF normal, tau resample, bounce cap
 //
 // A march step of length d consumes d*kappa of optical depth (:64-83) and walks the voxel grid
-// with an integer DDA (Amanatides & Woo 1987) adding each chord to the FP64 fluence grid — the
-// path-length estimator of `update_grids` (:213-292).  When the flight's optical depth runs out
-// the albedo test absorbs or scatters (HG, src/photon.f90:713-779) and a new flight starts.
+// with an integer DDA (Amanatides & Woo 1987), adding each chord to the FP64 fluence grid — the
+// path-length estimator of `update_grids` (:213-292).  A walk advances at most `dda_cap` chords
+// per iteration and resumes in the next one, so a few long walks do not stall the wavefront.
+// Consecutive chords in the same voxel (a step ending and the next one starting in it) are
+// merged before the FP64 atomic is issued.  When the flight's optical depth runs out, the
+// albedo test absorbs or scatters (HG, src/photon.f90:713-779) and a new flight starts.
 // A lane whose packet ends takes the next packet id from its wavefront's chunk at the top of
-// the next iteration (regeneration); a wavefront claims a new chunk of ids from a global
-// counter with one atomic.  Packet j draws from Philox stream (seed, first_packet + j), so
-// results do not depend on grid size, block size or chunking.
+// the next iteration (regeneration); a wavefront claims a chunk of ids from a global counter
+// with one atomic.  Packet j draws from Philox stream (seed, first_packet + j), so results do
+// not depend on grid size, block size, chunking or the walk cap.
 #pragma once
 #include "smcrt_device.h"
 
@@ -33,8 +37,11 @@ enum : int {
     O_LAUNCHED = 0, O_ABSORBED, O_R, O_T, O_KSURF, O_KBOUNCE, O_SCATTERS, NOUT
 };
 enum : int {
-    E_SWEEPS = 0, E_REGION, E_NORMALS, E_STEPS, E_CROSS, E_IFACE, E_FRESNEL, E_UNREACH, E_NAN, NEV
+    E_SWEEPS = 0, E_REGION, E_NORMALS, E_STEPS, E_CROSS, E_IFACE, E_FRESNEL, E_UNREACH, E_NAN, E_ATOMICS, NEV
 };
+
+// most region-indexed material entries staged in LDS (nsets * nregions)
+#define SMCRT_LDS_MATS 2048
 
 struct KParams {
     const DevNode* nodes;
@@ -51,12 +58,12 @@ struct KParams {
     int src_kind, circ_swap;
     float spos[3], sdir[3], p1[3], p2[3], p3[3], radius;
     float circ_t[12];
-    float rdir[3];  // R/T classification direction
+    float rdir[3];   // R/T classification direction
     float eps;       // march / crossing epsilon
     float eps_kill;  // flight-start kill threshold after emission or scatter
     // run
     uint32_t seed_lo, seed_hi;
-    unsigned long long total;          // work items (sum of set budgets)
+    unsigned long long total;              // work items (sum of set budgets)
     int nsets;
     const unsigned long long* set_prefix;  // nsets+1 cumulative budgets
     const unsigned long long* set_first;   // global packet id of each set's first packet
@@ -66,24 +73,33 @@ struct KParams {
     float* trace;
     int trace_packets, trace_events;
     uint32_t flags;
-    int max_bounces, chunk;
+    int max_bounces, chunk, dda_cap;
+    // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP32 LDS per block
+    // and are flushed into the FP64 grid at the end (tn[0] == 0: no tile)
+    int t0[3], tn[3];
     // adjoint
     const float* weights;  // nullptr: w = 1
     double* grad;          // [4][nmat] + [1] weighted tally
-    const int* region_mat; // region -> adjoint material index
     const float* mat_mus;  // mus per adjoint material
 };
 
 struct Lane {
     float px, py, pz, dx, dy, dz;
     float opx, opy, opz;  // position before the crossing step
-    float tau, taurun, dlay_old;
+    float tau, taurun, dlay_old, dprev;
     int layer, st, bounces, nsc, set, tcount;
-    float dprev;     // distance at the previous march step
-    bool iface;      // the current flight started at an interface
+    bool iface;           // the current flight started at an interface
     unsigned iters;
     unsigned long long pid;
     Rng rng;
+};
+
+// voxel walk in progress
+struct Walk {
+    int ix, iy, iz, idx;
+    float tmx, tmy, tmz, t, d;
+    bool active, flight_over, inside;
+    float aint;  // adjoint: integral of C over the step
 };
 
 SMCRT_DEV float rnd(Lane& L, const KParams& P) { return rng_u(L.rng, P.seed_lo, P.seed_hi); }
@@ -91,7 +107,7 @@ SMCRT_DEV float rnd(Lane& L, const KParams& P) { return rng_u(L.rng, P.seed_lo, 
 SMCRT_DEV uint64_t wave_ballot(bool b) { return __ballot(b); }
 
 SMCRT_DEV void trace_event(const KParams& P, Lane& L, int kind) {
-    if (L.pid < (unsigned long long)P.trace_packets && L.tcount < P.trace_events) {
+    if (P.trace && L.pid < (unsigned long long)P.trace_packets && L.tcount < P.trace_events) {
         float* e = P.trace + ((size_t)L.pid * P.trace_events + L.tcount) * 4;
         e[0] = L.px; e[1] = L.py; e[2] = L.pz; e[3] = (float)kind;
         L.tcount++;
@@ -185,30 +201,21 @@ SMCRT_DEV float fresnel(float ix, float iy, float iz, float nx, float ny, float 
     return 0.5f * (f1 * f1 + f2 * f2);
 }
 
-// Evaluate, per lane, the region r_lane at p (only lanes with active set).  Waterfall over the
+// Evaluate, per lane, the region r_lane at p (only lanes with `active` set).  Waterfall over the
 // distinct region ids in the wavefront so each evaluation runs the wave-uniform interpreter.
 template <class T>
-SMCRT_DEV T eval_lane_region(const KParams& P, bool active, int r_lane, const V3<T>& p) {
+SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane, const V3<T>& p) {
     T res = g_const<T>(0.f);
     uint64_t pending = wave_ballot(active);
     while (pending) {
         int leader = __builtin_ctzll(pending);
         int r = __builtin_amdgcn_readlane(r_lane, leader);
         bool mine = active && r_lane == r;
-        if (mine) {
-            const DevRegion reg = P.regions[r];
-            res = region_eval<T>(P.nodes, reg.first, reg.count, p);
-        }
+        if (mine) res = region_eval<T>(nodes, regions[r].first, regions[r].count, p);
         pending &= ~wave_ballot(mine);
     }
     return res;
 }
-
-template <bool ADJ>
-struct AdjState {
-    float C;      // weighted in-grid path so far
-    double wsum;  // sum of C over ended packets
-};
 
 // LDS layout for the adjoint: scores float [4][nmat][blockDim], gradients double [4][nmat][blockDim]
 struct AdjLds {
@@ -224,78 +231,55 @@ SMCRT_DEV double& adj_g(const AdjLds& A, int which, int m, int nmat) {
     return A.g[(which * nmat + m) * A.stride + threadIdx.x];
 }
 
-// Walk the voxels along a step of length d from the lane's position and tally the chords.
-// Returns false if the step leaves the grid (the packet then ends, src/inttau2.f90:256-285);
-// *in_grid receives the tallied length.  For the adjoint, accumulates C and the step integral
-// of C into *cacc / *aint.
-template <bool ADJ>
-SMCRT_DEV bool dda(const KParams& P, double* __restrict__ grid, const Lane& L, float d,
-                   float* in_grid, unsigned& crossings, float* cacc, float* aint) {
-    float gx = (L.px + P.xmax) * P.sx, gy = (L.py + P.ymax) * P.sy, gz = (L.pz + P.zmax) * P.sz;
-    int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
-    *in_grid = 0.f;
-    if ((unsigned)ix >= (unsigned)P.nx || (unsigned)iy >= (unsigned)P.ny || (unsigned)iz >= (unsigned)P.nz)
+SMCRT_DEV int classify_escape(const KParams& P, const Lane& L) {
+    return (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
+}
+
+// Set up the voxel walk of a step of length d from the lane's position.  Returns false when the
+// start lies outside the grid (the packet ends there, src/inttau2.f90:256-259).
+SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
+    const float gx = (L.px + P.xmax) * P.sx, gy = (L.py + P.ymax) * P.sy, gz = (L.pz + P.zmax) * P.sz;
+    W.ix = (int)floorf(gx);
+    W.iy = (int)floorf(gy);
+    W.iz = (int)floorf(gz);
+    W.t = 0.f;
+    W.d = d;
+    W.aint = 0.f;
+    if ((unsigned)W.ix >= (unsigned)P.nx || (unsigned)W.iy >= (unsigned)P.ny || (unsigned)W.iz >= (unsigned)P.nz)
         return false;
     const float INF = __builtin_inff();
-    float adx = fabsf(L.dx) * P.sx, ady = fabsf(L.dy) * P.sy, adz = fabsf(L.dz) * P.sz;
-    float tdx = adx > 0.f ? f_rcp(adx) : INF;
-    float tdy = ady > 0.f ? f_rcp(ady) : INF;
-    float tdz = adz > 0.f ? f_rcp(adz) : INF;
-    float tmx = adx > 0.f ? (L.dx > 0.f ? (float)(ix + 1) - gx : gx - (float)ix) * tdx : INF;
-    float tmy = ady > 0.f ? (L.dy > 0.f ? (float)(iy + 1) - gy : gy - (float)iy) * tdy : INF;
-    float tmz = adz > 0.f ? (L.dz > 0.f ? (float)(iz + 1) - gz : gz - (float)iz) * tdz : INF;
-    const int sxs = L.dx > 0.f ? 1 : -1, sys = L.dy > 0.f ? 1 : -1, szs = L.dz > 0.f ? 1 : -1;
-    const int nxny = P.nx * P.ny;
-    int idx = ix + P.nx * (iy + P.ny * iz);
-    float t = 0.f;
-    bool inside = true;
-    for (;;) {
-        float tn = fminf(tmx, fminf(tmy, tmz));
-        float end = fminf(tn, d);
-        float len = end - t;
-        if (len > 0.f) {
-            if (grid) unsafeAtomicAdd(grid + idx, (double)len);
-            crossings++;
-            if (ADJ) {
-                float w = P.weights ? P.weights[idx] : 1.f;
-                *aint += *cacc * len + 0.5f * w * len * len;
-                *cacc += w * len;
-            }
-        }
-        if (tn >= d) break;
-        t = tn;
-        if (tmx <= tmy && tmx <= tmz) {
-            ix += sxs;
-            if ((unsigned)ix >= (unsigned)P.nx) { inside = false; break; }
-            idx += sxs;
-            tmx += tdx;
-        } else if (tmy <= tmz) {
-            iy += sys;
-            if ((unsigned)iy >= (unsigned)P.ny) { inside = false; break; }
-            idx += sys * P.nx;
-            tmy += tdy;
-        } else {
-            iz += szs;
-            if ((unsigned)iz >= (unsigned)P.nz) { inside = false; break; }
-            idx += szs * nxny;
-            tmz += tdz;
-        }
-    }
-    *in_grid = inside ? d : t;
-    return inside;
+    const float adx = fabsf(L.dx) * P.sx, ady = fabsf(L.dy) * P.sy, adz = fabsf(L.dz) * P.sz;
+    W.tmx = adx > 0.f ? (L.dx > 0.f ? (float)(W.ix + 1) - gx : gx - (float)W.ix) * f_rcp(adx) : INF;
+    W.tmy = ady > 0.f ? (L.dy > 0.f ? (float)(W.iy + 1) - gy : gy - (float)W.iy) * f_rcp(ady) : INF;
+    W.tmz = adz > 0.f ? (L.dz > 0.f ? (float)(W.iz + 1) - gz : gz - (float)W.iz) * f_rcp(adz) : INF;
+    W.idx = W.ix + P.nx * (W.iy + P.ny * W.iz);
+    return true;
 }
 
 template <bool ADJ>
 __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    CDevNode* const nodes = (CDevNode*)P.nodes;
+    CDevRegion* const regions = (CDevRegion*)P.regions;
+
+    // region-indexed material tables of every set, staged in LDS
+    DevMat* const s_mat = reinterpret_cast<DevMat*>(smem);
+    const int n_mats = P.nsets * P.nreg;
+    for (int i = threadIdx.x; i < n_mats; i += blockDim.x) s_mat[i] = P.mats[i];
+    // the tally tile follows the materials
+    float* const s_tile = reinterpret_cast<float*>(smem + sizeof(DevMat) * n_mats);
+    const int tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
+    for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) s_tile[i] = 0.f;
     AdjLds A;
     if (ADJ) {
         A.stride = blockDim.x;
-        A.sc = reinterpret_cast<float*>(smem);
-        A.g = reinterpret_cast<double*>(smem + sizeof(float) * 4 * P.nmat * blockDim.x);
+        unsigned char* base = smem + sizeof(DevMat) * n_mats + sizeof(float) * ((tile_vox + 3) & ~3);
+        A.sc = reinterpret_cast<float*>(base);
+        A.g = reinterpret_cast<double*>(base + sizeof(float) * 4 * P.nmat * blockDim.x);
         for (int w = 0; w < 4; ++w)
             for (int m = 0; m < P.nmat; ++m) adj_g(A, w, m, P.nmat) = 0.0;
     }
+    __syncthreads();
     float adjC = 0.f;
     double adjW = 0.0;
 
@@ -307,10 +291,15 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     L.iface = false;
     L.dprev = 0.f;
     L.iters = 0;
+    Walk W;
+    W.active = false;
+    int pend_idx = -1;  // merged chord waiting for its atomic (grid index)
+    int pend_li = -1;   // its tile index, -1 outside the tile
+    float pend_len = 0.f;
     unsigned out[NOUT];
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) out[i] = 0;
-    unsigned ev[NEV];
+    unsigned ev[NEV];  // wave-uniform except steps/crossings/atomics/errors (per lane)
 #pragma unroll
     for (int i = 0; i < NEV; ++i) ev[i] = 0;
     int cur_set = 0;
@@ -318,9 +307,10 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     unsigned long long base = 0;
     unsigned avail = 0;
     bool exhausted = false;
-    const float eps = P.eps;          // march / crossing epsilon (FP32-safe)
-    const float eps_kill = P.eps_kill; // flight-start kill threshold after emission or scatter
+    const float eps = P.eps;
+    const float eps_kill = P.eps_kill;
     const unsigned chunk = (unsigned)P.chunk;
+    const int cap = P.dda_cap;
     const int lane_id = __lane_id();
 
     for (;;) {
@@ -372,8 +362,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             continue;
         }
 
-        const bool busy = L.st != ST_IDLE;
-        const DevMat* __restrict__ mt = P.mats + (size_t)L.set * P.nreg;
+        const DevMat* mt = s_mat + L.set * P.nreg;
         double* __restrict__ grid = P.jmean ? P.jmean + (size_t)L.set * P.grid_stride : nullptr;
 
         // ---- emission ----
@@ -393,15 +382,16 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             trace_event(P, L, 1);
         }
 
-        // ---- the sweep: every region at the lane's position ----
+        // ---- the sweep: every region at the lane's position (lanes not mid-walk) ----
+        const bool sweeping = L.st != ST_IDLE && !W.active;
         float dmin = __builtin_inff(), smin = __builtin_inff(), dfirst = 0.f, dlay = 0.f, d0 = 0.f;
         float bestv = __builtin_inff();
         int fn = -1, best = 0;
-        if (busy) {
+        bool runaway = false;
+        if (sweeping) {
             V3<float> p{L.px, L.py, L.pz};
             for (int r = 0; r < P.nreg; ++r) {
-                const DevRegion reg = P.regions[r];
-                float d = region_eval<float>(P.nodes, reg.first, reg.count, p);
+                float d = region_eval<float>(nodes, regions[r].first, regions[r].count, p);
                 dmin = fminf(dmin, fabsf(d));
                 smin = fminf(smin, d);
                 if (r == 0) d0 = d;
@@ -410,15 +400,12 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 float c = d > 0.f ? 999.f : -d;  // positive distances count as -999
                 if (c < bestv) { bestv = c; best = r; }
             }
-            ev[E_SWEEPS]++;
             if (fn < 0) { fn = 0; dfirst = d0; }  // minloc of an all-zero array is index 1
-            if (++L.iters > (1u << 24)) {         // runaway guard: never expected
-                ev[E_NAN]++;
-                L.st = ST_IDLE;
-            }
+            runaway = ++L.iters > (1u << 24);     // runaway guard: never expected
         }
+        ev[E_SWEEPS] += __builtin_popcountll(wave_ballot(sweeping));
 
-        if (L.st == ST_EMIT) {
+        if (sweeping && L.st == ST_EMIT) {
             L.layer = best;  // nearest containing surface (src/kernelsMod.f90:275-280)
             L.tau = f_neglog(rnd(L, P));
             L.taurun = 0.f;
@@ -426,22 +413,23 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             L.st = ST_START;
         }
 
-        int end_kind = -1;  // >= 0: the packet ends this iteration
-        bool want_dstmp = false, want_normal = false, do_march = false, cross = false;
+        int end_kind = runaway ? END_ERR : -1;  // >= 0: the packet ends this iteration
+        if (runaway) ev[E_NAN]++;
+        bool want_dstmp = false, want_normal = false, do_march = false, cross = false, fres = false;
         int use_region = 0;
-        const int st0 = L.st;
+        const int st0 = (sweeping && !runaway) ? L.st : ST_IDLE;
         if (st0 == ST_START) {
-            // src/inttau2.f90:52-63.  After an interface the threshold is the march epsilon (the
-            // crossing-step ratio that kills grazing crossings and reflected packets); after
-            // emission or a scatter it is the reference's own 1e-8.
             // FP32 distances are quantised (multiples of 2^-24 near |p| = 1), so d == eps ties
             // that never occur in the reference's FP64 are common here: a tie counts as inside
             // eps, which kills a reflected packet (pos = oldpos, d0 <= eps) as the reference does.
+            // After an interface the threshold is the march epsilon (the crossing-step ratio
+            // that kills grazing crossings and reflected packets); after emission or a scatter
+            // it is the reference's own 1e-8 (src/inttau2.f90:58-61).
             if (L.iface ? (dmin <= eps) : (dmin < eps_kill)) end_kind = END_SURFACE;
             else do_march = true;
         } else if (st0 == ST_MARCHED) {
             // src/inttau2.f90:85-97
-            if (smin >= 0.f) end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
+            if (smin >= 0.f) end_kind = classify_escape(P, L);
             else if (dmin > eps) do_march = true;
             else if (dmin <= L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
             else do_march = true;  // within eps but moving away from the surface: keep marching
@@ -450,7 +438,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             const int nl = fn;
             const float n1 = mt[L.layer].n, n2 = mt[nl].n;
             if (n1 != n2) {
-                ev[E_FRESNEL]++;
+                fres = true;
                 const float dl = dlay, dn = dfirst;
                 if (dl < 0.f && dn < 0.f) {
                     use_region = fabsf(dn) < fabsf(dl) ? nl : L.layer;
@@ -466,10 +454,10 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     use_region = L.layer;
                     want_normal = true;
                 } else if (dl > 0.f && dn > 0.f) {
-                    end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
+                    end_kind = classify_escape(P, L);
                 } else {
+                    end_kind = END_ERR;  // "This should not be reached!" (:156)
                     ev[E_UNREACH]++;
-                    end_kind = END_ERR;
                 }
             } else {
                 L.layer = nl;  // index-matched: switch region, keep tau (:182-184)
@@ -477,23 +465,24 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 L.st = ST_START;
             }
         }
+        ev[E_FRESNEL] += __builtin_popcountll(wave_ballot(fres));
 
         if (cross) {
             // interface: step a little past the surface (src/inttau2.f90:104-116)
-            ev[E_IFACE]++;
             L.opx = L.px; L.opy = L.py; L.opz = L.pz;
             L.dlay_old = dlay;
             const float s = dmin + 2.f * eps;
             L.px += s * L.dx; L.py += s * L.dy; L.pz += s * L.dz;
             L.st = ST_CROSSED;
         }
+        ev[E_IFACE] += __builtin_popcountll(wave_ballot(cross));
 
         // dstmp(new): region nl at the pre-crossing position (waterfall over distinct regions)
-        if (wave_ballot(want_dstmp)) {
+        if (const uint64_t m = wave_ballot(want_dstmp)) {
+            ev[E_REGION] += __builtin_popcountll(m);
             V3<float> op{L.opx, L.opy, L.opz};
-            float dt = eval_lane_region<float>(P, want_dstmp, fn, op);
+            float dt = eval_lane_region<float>(nodes, regions, want_dstmp, fn, op);
             if (want_dstmp) {
-                ev[E_REGION]++;
                 const float dl = dlay, dn = dfirst;
                 if (dt < 0.f) {
                     use_region = dt > L.dlay_old ? fn : L.layer;
@@ -502,20 +491,20 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     use_region = L.layer;
                     want_normal = true;
                 } else if (dl > 0.f && dn > 0.f) {
-                    end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
+                    end_kind = classify_escape(P, L);
                 } else {
+                    end_kind = END_ERR;  // "This should not be reached!" (:156)
                     ev[E_UNREACH]++;
-                    end_kind = END_ERR;
                 }
             }
         }
 
         // normal + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
-        if (wave_ballot(want_normal)) {
+        if (const uint64_t m = wave_ballot(want_normal)) {
+            ev[E_NORMALS] += __builtin_popcountll(m);
             V3<Dual> pd{Dual{L.px, 1.f, 0.f, 0.f}, Dual{L.py, 0.f, 1.f, 0.f}, Dual{L.pz, 0.f, 0.f, 1.f}};
-            Dual g = eval_lane_region<Dual>(P, want_normal, use_region, pd);
+            Dual g = eval_lane_region<Dual>(nodes, regions, want_normal, use_region, pd);
             if (want_normal) {
-                ev[E_NORMALS]++;
                 float inv = f_rsqrt(g.x * g.x + g.y * g.y + g.z * g.z);
                 float nx = g.x * inv, ny = g.y * inv, nz = g.z * inv;
                 const int nl = fn;
@@ -549,39 +538,100 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             }
         }
 
-        // ---- march step (src/inttau2.f90:64-83) ----
+        // ---- march step (src/inttau2.f90:64-83): optical depth, then the voxel walk ----
         if (do_march) {
             ev[E_STEPS]++;
             L.dprev = dmin;
             const float kap = mt[L.layer].kappa;
             const float t = dmin * kap;
-            const bool flight_over = !(L.taurun + t <= L.tau);
+            W.flight_over = !(L.taurun + t <= L.tau);
             float d = dmin;
-            if (flight_over) d = (L.tau - L.taurun) / kap;
+            if (W.flight_over) d = (L.tau - L.taurun) / kap;
             else L.taurun += t;
-            float in_grid = 0.f, aint = 0.f;
-            const bool inside = dda<ADJ>(P, grid, L, d, &in_grid, ev[E_CROSS], &adjC, &aint);
+            W.inside = walk_begin(P, L, W, d);
+            W.active = true;
+        }
+
+        // ---- voxel walk: at most `cap` chords per lane this iteration ----
+        bool walk_done = false;
+        if (W.active) {
+            if (W.inside) {
+                for (int k = 0; k < cap; ++k) {
+                    const float tn = fminf(W.tmx, fminf(W.tmy, W.tmz));
+                    const float len = fminf(tn, W.d) - W.t;
+                    if (len > 0.f) {
+                        ev[E_CROSS]++;
+                        if (ADJ) {
+                            const float w = P.weights ? P.weights[W.idx] : 1.f;
+                            W.aint += adjC * len + 0.5f * w * len * len;
+                            adjC += w * len;
+                        }
+                        if (grid) {
+                            if (W.idx == pend_idx) {
+                                pend_len += len;
+                            } else {
+                                if (pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
+                                else if (pend_idx >= 0) { unsafeAtomicAdd(grid + pend_idx, (double)pend_len); ev[E_ATOMICS]++; }
+                                pend_idx = W.idx;
+                                pend_len = len;
+                                const unsigned lx = (unsigned)(W.ix - P.t0[0]), ly = (unsigned)(W.iy - P.t0[1]),
+                                               lz = (unsigned)(W.iz - P.t0[2]);
+                                pend_li = (lx < (unsigned)P.tn[0] && ly < (unsigned)P.tn[1] && lz < (unsigned)P.tn[2])
+                                              ? (int)(lx + P.tn[0] * (ly + P.tn[1] * lz)) : -1;
+                            }
+                        }
+                    }
+                    if (tn >= W.d) { walk_done = true; break; }
+                    W.t = tn;
+                    if (W.tmx <= W.tmy && W.tmx <= W.tmz) {
+                        const int s = L.dx > 0.f ? 1 : -1;
+                        W.ix += s;
+                        if ((unsigned)W.ix >= (unsigned)P.nx) { W.inside = false; walk_done = true; break; }
+                        W.idx += s;
+                        W.tmx += f_rcp(fabsf(L.dx) * P.sx);
+                    } else if (W.tmy <= W.tmz) {
+                        const int s = L.dy > 0.f ? 1 : -1;
+                        W.iy += s;
+                        if ((unsigned)W.iy >= (unsigned)P.ny) { W.inside = false; walk_done = true; break; }
+                        W.idx += s * P.nx;
+                        W.tmy += f_rcp(fabsf(L.dy) * P.sy);
+                    } else {
+                        const int s = L.dz > 0.f ? 1 : -1;
+                        W.iz += s;
+                        if ((unsigned)W.iz >= (unsigned)P.nz) { W.inside = false; walk_done = true; break; }
+                        W.idx += s * P.nx * P.ny;
+                        W.tmz += f_rcp(fabsf(L.dz) * P.sz);
+                    }
+                }
+            } else {
+                walk_done = true;  // started outside the grid
+            }
+        }
+
+        // ---- step finished: move, then end of flight (albedo test) or next march step ----
+        if (walk_done) {
+            W.active = false;
+            const float d = W.d;
             if (ADJ) {
-                const int m = P.region_mat[L.layer];
-                if (inside) {
+                const int m = mt[L.layer].mat;
+                if (W.inside) {
                     adj_sc(A, 0, m, P.nmat) += d;
-                    adj_sc(A, 1, m, P.nmat) += aint;
+                    adj_sc(A, 1, m, P.nmat) += W.aint;
                 } else {  // left the grid: the score stops at the exit point
-                    adj_sc(A, 0, m, P.nmat) += in_grid;
-                    adj_sc(A, 1, m, P.nmat) += aint;
+                    adj_sc(A, 0, m, P.nmat) += W.t;
+                    adj_sc(A, 1, m, P.nmat) += W.aint;
                 }
             }
             L.px += d * L.dx; L.py += d * L.dy; L.pz += d * L.dz;
-            if (!inside) {
-                end_kind = (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
-            } else if (flight_over) {
+            if (!W.inside) {
+                end_kind = classify_escape(P, L);
+            } else if (W.flight_over) {
                 // albedo test (src/kernelsMod.f90:287-297)
                 const DevMat m = mt[L.layer];
                 if (rnd(L, P) < m.albedo) {
                     if (ADJ) {
-                        const int am = P.region_mat[L.layer];
-                        adj_sc(A, 2, am, P.nmat) += 1.f;
-                        adj_sc(A, 3, am, P.nmat) += adjC;
+                        adj_sc(A, 2, m.mat, P.nmat) += 1.f;
+                        adj_sc(A, 3, m.mat, P.nmat) += adjC;
                     }
                     L.nsc++;
                     out[O_SCATTERS]++;
@@ -610,18 +660,27 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         }
 
         // non-finite guard
-        if (end_kind < 0 && L.st != ST_IDLE && !(fabsf(L.px) < 1e30f && fabsf(L.py) < 1e30f && fabsf(L.pz) < 1e30f)) {
+        if (end_kind < 0 && L.st != ST_IDLE && !W.active &&
+            !(fabsf(L.px) < 1e30f && fabsf(L.py) < 1e30f && fabsf(L.pz) < 1e30f)) {
             ev[E_NAN]++;
             end_kind = END_ERR;
         }
         if (end_kind >= 0) {
+            if (pend_li >= 0) {  // flush the merged chord before the lane changes packet/set
+                atomicAdd(s_tile + pend_li, pend_len);
+            } else if (pend_idx >= 0) {
+                unsafeAtomicAdd(grid + pend_idx, (double)pend_len);
+                ev[E_ATOMICS]++;
+            }
+            pend_idx = -1;
+            pend_li = -1;
             switch (end_kind) {
             case END_ABSORB: out[O_ABSORBED]++; trace_event(P, L, 5); break;
             case END_R: out[O_R]++; trace_event(P, L, 6); break;
             case END_T: out[O_T]++; trace_event(P, L, 6); break;
             case END_SURFACE: out[O_KSURF]++; trace_event(P, L, 7); break;
             case END_BOUNCE: out[O_KBOUNCE]++; trace_event(P, L, 7); break;
-            default: break;
+            default: break;  // error paths are counted where they occur
             }
             if (ADJ) {
                 const bool isr = end_kind == END_R;
@@ -645,13 +704,29 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         }
     }
 
+    // ---- flush the tally tile (single-set launches only) into the FP64 grid ----
+    if (tile_vox > 0) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) {
+            const float v = s_tile[i];
+            if (v != 0.f) {
+                const int lx = i % P.tn[0], r = i / P.tn[0];
+                const int ly = r % P.tn[1], lz = r / P.tn[1];
+                const size_t g = (size_t)(P.t0[0] + lx) + (size_t)P.nx * ((size_t)(P.t0[1] + ly) + (size_t)P.ny * (P.t0[2] + lz));
+                unsafeAtomicAdd(P.jmean + g, (double)v);
+            }
+        }
+    }
+
     // ---- flush counters ----
     for (int i = 0; i < NOUT; ++i)
         if (out[i]) atomicAdd(P.stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
     unsigned long long* evg = P.stats + (size_t)P.nsets * NOUT;
     for (int i = 0; i < NEV; ++i) {
         unsigned long long v = ev[i];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        const bool per_lane = (i == E_STEPS || i == E_CROSS || i == E_ATOMICS || i == E_NAN || i == E_UNREACH);
+        if (per_lane)
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane_id == 0 && v) atomicAdd(evg + i, v);
     }
     if (ADJ) {

@@ -60,9 +60,10 @@ __global__ void k_sum_f64(const double* __restrict__ p, size_t n, double* out) {
 
 // SDF values / exact gradients of one region at n points (one point per thread; the region's
 // program is uniform across the grid).
-__global__ void k_evaluate(const DevNode* __restrict__ nodes, DevRegion reg, long long n,
+__global__ void k_evaluate(const DevNode* __restrict__ gnodes, DevRegion reg, long long n,
                            const float* __restrict__ pts, float* __restrict__ dist,
                            float* __restrict__ nrm) {
+    CDevNode* nodes = (CDevNode*)gnodes;
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
@@ -116,6 +117,9 @@ struct smcrt_scene {
     float circ_t[12];
     int circ_swap;
     float rdir[3];
+    // LDS tally tile (voxel box), chosen from a pilot run on first use
+    bool tile_ready = false;
+    int tile_t0[3] = {0, 0, 0}, tile_tn[3] = {0, 0, 0};
     // device
     int device = -1;
     int num_cu = 0;
@@ -146,7 +150,7 @@ static void region_mats(const smcrt_scene* s, const smcrt_material* mats, DevMat
         d.n = (float)m.n;
         d.mus = (float)m.mus;
         d.mua = (float)m.mua;
-        d.pad = 0.f;
+        d.mat = s->region_mat[r];
         out[r] = d;
     }
 }
@@ -427,38 +431,107 @@ static void fill_common(const smcrt_scene* s, KParams& P) {
     memcpy(P.circ_t, s->circ_t, sizeof P.circ_t);
     P.eps = s->eps;
     P.eps_kill = s->eps_kill;
-    P.region_mat = s->d_region_mat.as<int>();
     P.mat_mus = s->d_mat_mus.as<float>();
 }
 
 struct LaunchShape {
-    int block, blocks, chunk;
+    int block, blocks, chunk, walk_cap;
     size_t lds;
 };
 
-static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj) {
+// Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
+// material tables of n_sets sets, plus the adjoint's per-lane scores and gradients).
+static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0) {
     LaunchShape L;
     L.block = (o && o->block_size > 0) ? o->block_size : 256;
     int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
-    L.lds = 0;
-    if (adj) {
-        const size_t per_lane = (size_t)48 * s->mats.size();
-        while (L.block > 64 && per_lane * L.block > 65536) L.block /= 2;
-        L.lds = per_lane * L.block;
-    }
+    L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
+    const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
+    L.lds = mats;
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
     if (L.block > 256) L.block = 256;
+    if (adj) {
+        const size_t per_lane = (size_t)48 * s->mats.size();
+        while (L.block > 64 && mats + per_lane * L.block > 65536) L.block /= 2;
+        L.lds = mats + per_lane * L.block;
+    }
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once
         int active = 0;
         hipError_t e = adj ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<true>, L.block, L.lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<false>, L.block, 0);
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<false>, L.block, L.lds);
         if (e == hipSuccess && active > 0) bpc = active;
     }
     L.blocks = s->num_cu * bpc;
     return L;
+}
+
+// Greedy voxel box of at most `budget` voxels holding as much of `g`'s mass as possible: start
+// at the heaviest voxel and repeatedly extend by the face slab with the highest mass per voxel.
+static void choose_tile(const double* g, int nx, int ny, int nz, int budget, int t0[3], int tn[3]) {
+    const size_t nv = (size_t)nx * ny * nz;
+    size_t best = 0;
+    for (size_t i = 1; i < nv; ++i)
+        if (g[i] > g[best]) best = i;
+    int lo[3] = {(int)(best % nx), (int)((best / nx) % ny), (int)(best / ((size_t)nx * ny))};
+    int hi[3] = {lo[0] + 1, lo[1] + 1, lo[2] + 1};
+    const int n[3] = {nx, ny, nz};
+    auto at = [&](int i, int j, int k) { return g[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)]; };
+    for (;;) {
+        const long vol = (long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+        double best_d = -1.0;
+        int best_f = -1;
+        for (int f = 0; f < 6; ++f) {
+            const int ax = f / 2, up = f % 2;
+            if (up ? hi[ax] >= n[ax] : lo[ax] <= 0) continue;
+            const long area = vol / (hi[ax] - lo[ax]);
+            if (vol + area > budget) continue;
+            const int c = up ? hi[ax] : lo[ax] - 1;
+            int b0[3] = {lo[0], lo[1], lo[2]}, b1[3] = {hi[0], hi[1], hi[2]};
+            b0[ax] = c;
+            b1[ax] = c + 1;
+            double m = 0.0;
+            for (int k = b0[2]; k < b1[2]; ++k)
+                for (int j = b0[1]; j < b1[1]; ++j)
+                    for (int i = b0[0]; i < b1[0]; ++i) m += at(i, j, k);
+            const double dens = m / (double)area;
+            if (dens > best_d) { best_d = dens; best_f = f; }
+        }
+        if (best_f < 0) break;
+        if (best_f % 2) hi[best_f / 2]++;
+        else lo[best_f / 2]--;
+    }
+    for (int a = 0; a < 3; ++a) { t0[a] = lo[a]; tn[a] = hi[a] - lo[a]; }
+}
+
+static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
+                      hipStream_t stream, float* kernel_ms);
+
+// Pick the LDS tally tile from a pilot run of the scene (once per scene).
+static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stream) {
+    if (s->tile_ready) return SMCRT_OK;
+    s->tile_ready = true;  // the pilot itself runs without a tile
+    const size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+    double* pilot = nullptr;
+    if (hipMalloc(&pilot, nv * sizeof(double)) != hipSuccess) return SMCRT_OK;  // no tile, not an error
+    k_fill_f64<<<(unsigned)fill_blocks(nv), 256, 0, stream>>>(pilot, nv);
+    smcrt_run_opts po = *o;
+    po.n_packets = 100000;
+    po.first_packet = (1ull << 62);
+    po.flags = 0;
+    po.trace_packets = 0;
+    po.trace_events = 0;
+    float ms = 0.f;
+    int rc = launch_run(s, &po, pilot, false, nullptr, stream, &ms);
+    std::vector<double> h(nv);
+    if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
+        // 64 KiB of FP32 LDS per block leaves room for two 256-lane blocks per CU
+        choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, 16384, s->tile_t0, s->tile_tn);
+    }
+    (void)hipFree(pilot);
+    return rc;
 }
 
 // Launch the packet kernel for one scene (one set).  d_jmean may be null (no tally).
@@ -469,6 +542,11 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
         return SMCRT_OK;
     }
     if (o->trace_packets < 0 || o->trace_events < 0) return fail(SMCRT_EINVAL, "negative trace sizes");
+    const bool use_tile = !adj && d_jmean && !(o->flags & SMCRT_FLAG_NO_LDS_TILE);
+    if (use_tile) {
+        int rc = ensure_tile(s, o, stream);
+        if (rc) return rc;
+    }
     KParams P;
     fill_common(s, P);
     P.jmean = d_jmean;
@@ -495,8 +573,14 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     }
     P.flags = o->flags;
     P.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
-    LaunchShape sh = pick_shape(s, o, adj);
+    int tile_vox = 0;
+    if (use_tile) {
+        for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[a]; P.tn[a] = s->tile_tn[a]; }
+        tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
+    }
+    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox);
     P.chunk = sh.chunk;
+    P.dda_cap = sh.walk_cap;
     P.weights = d_w;
     P.grad = s->d_grad.as<double>();
     k_fill_u64<<<1, 64, 0, stream>>>(P.work, 8);
@@ -505,7 +589,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     if (adj) k_fill_f64<<<1, 256, 0, stream>>>(P.grad, 4 * s->mats.size() + 1);
     HIPCHECK(hipEventRecord(s->ev0, stream));
     if (adj) hipLaunchKernelGGL(packet_kernel<true>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
-    else hipLaunchKernelGGL(packet_kernel<false>, dim3(sh.blocks), dim3(sh.block), 0, stream, P);
+    else hipLaunchKernelGGL(packet_kernel<false>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(s->ev1, stream));
     HIPCHECK(hipStreamSynchronize(stream));
@@ -536,6 +620,7 @@ static int read_stats(smcrt_scene* s, smcrt_stats* st, float kernel_ms) {
     st->ev_fresnel = e[E_FRESNEL];
     st->err_unreachable = e[E_UNREACH];
     st->err_nan = e[E_NAN];
+    st->ev_atomics = e[E_ATOMICS];
     memcpy(st->moments, mom, sizeof mom);
     st->kernel_ms = kernel_ms;
     return SMCRT_OK;
@@ -799,8 +884,14 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         P.stats = b->d_stats.as<unsigned long long>() + g.stats_off;
         P.moments = s->d_moments.as<double>();
         P.max_bounces = 1000;
-        g.sh = pick_shape(s, nullptr, false);
+        if (g.sets.size() * s->regions.size() > SMCRT_LDS_MATS) {
+            smcrt_batch_destroy(b);
+            return fail(SMCRT_ERANGE, "a geometry group holds %zu sets x %zu regions; at most %d material entries fit in LDS",
+                        g.sets.size(), s->regions.size(), SMCRT_LDS_MATS);
+        }
+        g.sh = pick_shape(s, nullptr, false, (int)g.sets.size());
         P.chunk = g.sh.chunk;
+        P.dda_cap = g.sh.walk_cap;
     }
     HIPCHECK(hipMemcpy(b->d_mats.p, hm.data(), nmats * sizeof(DevMat), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(b->d_info.p, info.data(), ninfo * sizeof(unsigned long long), hipMemcpyHostToDevice));
@@ -812,7 +903,7 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
     k_fill_u64<<<(unsigned)fill_blocks(nstats), 256, 0, b->stream>>>(b->d_stats.as<unsigned long long>(), nstats);
     k_fill_u64<<<1, 64, 0, b->stream>>>(b->d_work.as<unsigned long long>(), b->groups.size());
     for (BatchGroup& g : b->groups)
-        hipLaunchKernelGGL(packet_kernel<false>, dim3(g.sh.blocks), dim3(g.sh.block), 0, b->stream, g.P);
+        hipLaunchKernelGGL(packet_kernel<false>, dim3(g.sh.blocks), dim3(g.sh.block), g.sh.lds, b->stream, g.P);
     hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
     if (ce != hipSuccess) {
         smcrt_batch_destroy(b);

@@ -32,11 +32,19 @@ struct DevRegion {
     int first, count, material, pad;
 };
 
-// Per-material constants (one table per property set).
+// Per-region material constants (one table per property set); `mat` is the region's material
+// index (the adjoint's parameter index).
 struct alignas(16) DevMat {
     float kappa, albedo, hgg, g2;
-    float n, mus, mua, pad;
+    float n, mus, mua;
+    int mat;
 };
+
+// The scene program is wave-uniform: reading it through the constant address space lets the
+// compiler fetch it with scalar loads into SGPRs (the generic pointer would force vector loads,
+// since the kernel also writes global memory and the reads could otherwise be clobbered).
+typedef __attribute__((address_space(4))) const DevNode CDevNode;
+typedef __attribute__((address_space(4))) const DevRegion CDevRegion;
 
 // ------------------------------------------------------------------------------------------
 // float helpers
@@ -287,26 +295,34 @@ SMCRT_DEV V3<T> xform(const float* __restrict__ t, const V3<T>& p) {
 // so the opcode dispatch is a scalar branch and parameters come from scalar loads.  The value
 // and position stacks are shift registers (no dynamically indexed register arrays).
 template <class T>
-SMCRT_DEV T region_eval(const DevNode* __restrict__ nodes, int first, int count, const V3<T>& p0) {
+SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0) {
     T v0 = g_const<T>(0.f), v1 = v0, v2 = v0, v3 = v0;
     V3<T> q0 = p0, q1 = p0, q2 = p0;  // q0 = top of the position stack
     for (int i = 0; i < count; ++i) {
-        const DevNode* n = nodes + first + i;
-        const int op = __builtin_amdgcn_readfirstlane(n->op);
-        const int kind = __builtin_amdgcn_readfirstlane(n->kind);
+        CDevNode* n = nodes + first + i;
+        const int op = n->op;
+        const int kind = n->kind;
+        float pp[8], tt[12];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pp[j] = n->p[j];
         if (op == 1) {  // PRIM
-            V3<T> q = n->ident ? q0 : xform<T>(n->t, q0);
-            T d = prim_eval<T>(kind, n->p, q);
+            V3<T> q = q0;
+            if (!n->ident) {
+#pragma unroll
+                for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
+                q = xform<T>(tt, q0);
+            }
+            T d = prim_eval<T>(kind, pp, q);
             v3 = v2; v2 = v1; v1 = v0; v0 = d;
         } else if (op == 2) {  // CSG: d1 = v1, d2 = v0
             T d = csg_eval<T>(kind, v1, v0, n->k);
             v0 = d; v1 = v2; v2 = v3;
         } else if (op == 3) {  // MOD_BEGIN
-            V3<T> q = mod_pos<T>(kind, n->p, q0);
+            V3<T> q = mod_pos<T>(kind, pp, q0);
             q2 = q1; q1 = q0; q0 = q;
         } else {  // MOD_END
             q0 = q1; q1 = q2;
-            v0 = mod_post<T>(kind, n->p, q0, v0);
+            v0 = mod_post<T>(kind, pp, q0, v0);
         }
     }
     return v0;

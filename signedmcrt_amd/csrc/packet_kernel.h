@@ -43,6 +43,31 @@ enum : int {
 // most region-indexed material entries staged in LDS (nsets * nregions)
 #define SMCRT_LDS_MATS 2048
 
+// Parameters used off the hot path (emission, outcome classification, tracing, moments, the
+// adjoint), kept in device memory and read with scalar loads where they are used so that they
+// do not occupy SGPRs across the packet loop.
+struct KCold {
+    // source
+    int src_kind, circ_swap;
+    float spos[3], sdir[3], p1[3], p2[3], p3[3], radius;
+    float circ_t[12];
+    float rdir[3];  // R/T classification direction
+    // sets, counters, tracing
+    const unsigned long long* set_prefix;  // nsets+1 cumulative budgets
+    const unsigned long long* set_first;   // global packet id of each set's first packet
+    unsigned long long* stats;             // nsets*NOUT + NEV
+    double* moments;                       // 24 doubles
+    float* trace;
+    int trace_packets, trace_events;
+    uint32_t flags;
+    int max_bounces;
+    // adjoint
+    const float* weights;  // nullptr: w = 1
+    double* grad;          // [4][nmat] + [1] weighted tally
+    const float* mat_mus;  // mus per adjoint material
+};
+typedef __attribute__((address_space(4))) const KCold CKCold;
+
 struct KParams {
     const DevNode* nodes;
     const DevRegion* regions;
@@ -54,33 +79,18 @@ struct KParams {
     float sx, sy, sz;  // voxels per cm
     double* jmean;     // set 0 grid (nullptr: no tally)
     unsigned long long grid_stride;
-    // source
-    int src_kind, circ_swap;
-    float spos[3], sdir[3], p1[3], p2[3], p3[3], radius;
-    float circ_t[12];
-    float rdir[3];   // R/T classification direction
     float eps;       // march / crossing epsilon
     float eps_kill;  // flight-start kill threshold after emission or scatter
     // run
     uint32_t seed_lo, seed_hi;
     unsigned long long total;              // work items (sum of set budgets)
     int nsets;
-    const unsigned long long* set_prefix;  // nsets+1 cumulative budgets
-    const unsigned long long* set_first;   // global packet id of each set's first packet
     unsigned long long* work;              // work counter
-    unsigned long long* stats;             // nsets*NOUT + NEV
-    double* moments;                       // 24 doubles
-    float* trace;
-    int trace_packets, trace_events;
-    uint32_t flags;
-    int max_bounces, chunk, dda_cap;
+    int chunk, dda_cap;
     // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP32 LDS per block
     // and are flushed into the FP64 grid at the end (tn[0] == 0: no tile)
     int t0[3], tn[3];
-    // adjoint
-    const float* weights;  // nullptr: w = 1
-    double* grad;          // [4][nmat] + [1] weighted tally
-    const float* mat_mus;  // mus per adjoint material
+    const struct KCold* cold;  // rarely used parameters (constant memory)
 };
 
 struct Lane {
@@ -106,50 +116,50 @@ SMCRT_DEV float rnd(Lane& L, const KParams& P) { return rng_u(L.rng, P.seed_lo, 
 
 SMCRT_DEV uint64_t wave_ballot(bool b) { return __ballot(b); }
 
-SMCRT_DEV void trace_event(const KParams& P, Lane& L, int kind) {
-    if (P.trace && L.pid < (unsigned long long)P.trace_packets && L.tcount < P.trace_events) {
-        float* e = P.trace + ((size_t)L.pid * P.trace_events + L.tcount) * 4;
+SMCRT_DEV void trace_event(CKCold* C, Lane& L, int kind) {
+    if (C->trace && L.pid < (unsigned long long)C->trace_packets && L.tcount < C->trace_events) {
+        float* e = C->trace + ((size_t)L.pid * C->trace_events + L.tcount) * 4;
         e[0] = L.px; e[1] = L.py; e[2] = L.pz; e[3] = (float)kind;
         L.tcount++;
     }
 }
 
 // emitters (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338
-SMCRT_DEV void emit(const KParams& P, Lane& L) {
-    switch (P.src_kind) {
+SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
+    switch (C->src_kind) {
     case 2: {  // uniform: draws rx, ry
         float rx = rnd(L, P), ry = rnd(L, P);
-        L.px = P.p1[0] + rx * P.p2[0] + ry * P.p3[0];
-        L.py = P.p1[1] + rx * P.p2[1] + ry * P.p3[1];
-        L.pz = P.p1[2] + rx * P.p2[2] + ry * P.p3[2];
-        L.dx = P.sdir[0]; L.dy = P.sdir[1]; L.dz = P.sdir[2];
+        L.px = C->p1[0] + rx * C->p2[0] + ry * C->p3[0];
+        L.py = C->p1[1] + rx * C->p2[1] + ry * C->p3[1];
+        L.pz = C->p1[2] + rx * C->p2[2] + ry * C->p3[2];
+        L.dx = C->sdir[0]; L.dy = C->sdir[1]; L.dz = C->sdir[2];
         break;
     }
     case 3: {  // circular: draws radius, angle; disc rotated onto dir, centred at -position
-        float r = P.radius * f_sqrt(rnd(L, P));
+        float r = C->radius * f_sqrt(rnd(L, P));
         float th = rnd(L, P);
         float c = f_cos_rev(th), s = f_sin_rev(th);
         float qx, qy, qz;
-        if (P.circ_swap) { qx = r * c; qy = r * s; qz = 0.f; }
+        if (C->circ_swap) { qx = r * c; qy = r * s; qz = 0.f; }
         else { qx = 0.f; qy = r * c; qz = r * s; }
-        const float* t = P.circ_t;
+        const __attribute__((address_space(4))) float* t = C->circ_t;
         L.px = qx * t[0] + qy * t[3] + qz * t[6] + t[9];
         L.py = qx * t[1] + qy * t[4] + qz * t[7] + t[10];
         L.pz = qx * t[2] + qy * t[5] + qz * t[8] + t[11];
-        L.dx = P.sdir[0]; L.dy = P.sdir[1]; L.dz = P.sdir[2];
+        L.dx = C->sdir[0]; L.dy = C->sdir[1]; L.dz = C->sdir[2];
         break;
     }
     case 4: {  // point: draws azimuth, cos theta
         float ph = rnd(L, P);
         float ct = 2.f * rnd(L, P) - 1.f;
         float stt = f_sqrt(fmaxf(1.f - ct * ct, 0.f));
-        L.px = P.spos[0]; L.py = P.spos[1]; L.pz = P.spos[2];
+        L.px = C->spos[0]; L.py = C->spos[1]; L.pz = C->spos[2];
         L.dx = stt * f_cos_rev(ph); L.dy = stt * f_sin_rev(ph); L.dz = ct;
         break;
     }
     default: {  // pencil
-        L.px = P.spos[0]; L.py = P.spos[1]; L.pz = P.spos[2];
-        L.dx = P.sdir[0]; L.dy = P.sdir[1]; L.dz = P.sdir[2];
+        L.px = C->spos[0]; L.py = C->spos[1]; L.pz = C->spos[2];
+        L.dx = C->sdir[0]; L.dy = C->sdir[1]; L.dz = C->sdir[2];
         break;
     }
     }
@@ -231,8 +241,8 @@ SMCRT_DEV double& adj_g(const AdjLds& A, int which, int m, int nmat) {
     return A.g[(which * nmat + m) * A.stride + threadIdx.x];
 }
 
-SMCRT_DEV int classify_escape(const KParams& P, const Lane& L) {
-    return (L.dx * P.rdir[0] + L.dy * P.rdir[1] + L.dz * P.rdir[2] < 0.f) ? END_R : END_T;
+SMCRT_DEV int classify_escape(CKCold* C, const Lane& L) {
+    return (L.dx * C->rdir[0] + L.dy * C->rdir[1] + L.dz * C->rdir[2] < 0.f) ? END_R : END_T;
 }
 
 // Set up the voxel walk of a step of length d from the lane's position.  Returns false when the
@@ -260,6 +270,7 @@ template <bool ADJ>
 __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     CDevNode* const nodes = (CDevNode*)P.nodes;
+    CKCold* const C = (CKCold*)P.cold;
     CDevRegion* const regions = (CDevRegion*)P.regions;
 
     // region-indexed material tables of every set, staged in LDS
@@ -337,18 +348,18 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                         int lo = 0, hi = P.nsets - 1;
                         while (lo < hi) {
                             int mid = (lo + hi + 1) >> 1;
-                            if (P.set_prefix[mid] <= w) lo = mid; else hi = mid - 1;
+                            if (C->set_prefix[mid] <= w) lo = mid; else hi = mid - 1;
                         }
                         s = lo;
                     }
                     if (s != cur_set) {
                         for (int i = 0; i < NOUT; ++i)
-                            if (out[i]) atomicAdd(P.stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
+                            if (out[i]) atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
                         for (int i = 0; i < NOUT; ++i) out[i] = 0;
                         cur_set = s;
                     }
                     L.set = s;
-                    L.pid = P.set_first[s] + (w - P.set_prefix[s]);
+                    L.pid = C->set_first[s] + (w - C->set_prefix[s]);
                     L.st = ST_EMIT;
                 }
                 unsigned taken = cnt < avail ? cnt : avail;
@@ -368,7 +379,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         // ---- emission ----
         if (L.st == ST_EMIT) {
             rng_init(L.rng, L.pid);
-            emit(P, L);
+            emit(P, C, L);
             L.bounces = 0;
             L.nsc = 0;
             L.tcount = 0;
@@ -379,7 +390,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 for (int w = 0; w < 4; ++w)
                     for (int m = 0; m < P.nmat; ++m) adj_sc(A, w, m, P.nmat) = 0.f;
             }
-            trace_event(P, L, 1);
+            trace_event(C, L, 1);
         }
 
         // ---- the sweep: every region at the lane's position (lanes not mid-walk) ----
@@ -429,7 +440,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             else do_march = true;
         } else if (st0 == ST_MARCHED) {
             // src/inttau2.f90:85-97
-            if (smin >= 0.f) end_kind = classify_escape(P, L);
+            if (smin >= 0.f) end_kind = classify_escape(C, L);
             else if (dmin > eps) do_march = true;
             else if (dmin <= L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
             else do_march = true;  // within eps but moving away from the surface: keep marching
@@ -454,7 +465,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     use_region = L.layer;
                     want_normal = true;
                 } else if (dl > 0.f && dn > 0.f) {
-                    end_kind = classify_escape(P, L);
+                    end_kind = classify_escape(C, L);
                 } else {
                     end_kind = END_ERR;  // "This should not be reached!" (:156)
                     ev[E_UNREACH]++;
@@ -477,24 +488,41 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         }
         ev[E_IFACE] += __builtin_popcountll(wave_ballot(cross));
 
-        // dstmp(new): region nl at the pre-crossing position (waterfall over distinct regions)
-        if (const uint64_t m = wave_ballot(want_dstmp)) {
-            ev[E_REGION] += __builtin_popcountll(m);
-            V3<float> op{L.opx, L.opy, L.opz};
-            float dt = eval_lane_region<float>(nodes, regions, want_dstmp, fn, op);
-            if (want_dstmp) {
-                const float dl = dlay, dn = dfirst;
-                if (dt < 0.f) {
-                    use_region = dt > L.dlay_old ? fn : L.layer;
-                    want_normal = true;
-                } else if (dl > 0.f && dn < 0.f) {
-                    use_region = L.layer;
-                    want_normal = true;
-                } else if (dl > 0.f && dn > 0.f) {
-                    end_kind = classify_escape(P, L);
+        // Off-sweep region evaluations share one interpreter instance (single-direction duals):
+        // pass 0 = dstmp(new), region nl at the pre-crossing position; passes 1..3 = the gradient
+        // of the normal region, one seed direction per pass (exact, value + one partial).
+        float gxyz[3] = {0.f, 0.f, 0.f};
+        {
+            const uint64_t md = wave_ballot(want_dstmp);
+            ev[E_REGION] += __builtin_popcountll(md);
+#pragma nounroll
+            for (int pass = md ? 0 : 1; pass < 4; ++pass) {
+                if (pass == 1 && !wave_ballot(want_normal)) break;
+                const bool active = pass == 0 ? want_dstmp : want_normal;
+                const bool p0 = pass == 0;
+                V3<D1> pd{D1{p0 ? L.opx : L.px, pass == 1 ? 1.f : 0.f}, D1{p0 ? L.opy : L.py, pass == 2 ? 1.f : 0.f},
+                          D1{p0 ? L.opz : L.pz, pass == 3 ? 1.f : 0.f}};
+                const D1 r = eval_lane_region<D1>(nodes, regions, active, p0 ? fn : use_region, pd);
+                if (p0) {
+                    if (want_dstmp) {
+                        const float dt = r.v, dl = dlay, dn = dfirst;
+                        if (dt < 0.f) {
+                            use_region = dt > L.dlay_old ? fn : L.layer;
+                            want_normal = true;
+                        } else if (dl > 0.f && dn < 0.f) {
+                            use_region = L.layer;
+                            want_normal = true;
+                        } else if (dl > 0.f && dn > 0.f) {
+                            end_kind = classify_escape(C, L);
+                        } else {
+                            end_kind = END_ERR;  // "This should not be reached!" (:156)
+                            ev[E_UNREACH]++;
+                        }
+                    }
                 } else {
-                    end_kind = END_ERR;  // "This should not be reached!" (:156)
-                    ev[E_UNREACH]++;
+                    gxyz[0] = pass == 1 ? r.d : gxyz[0];
+                    gxyz[1] = pass == 2 ? r.d : gxyz[1];
+                    gxyz[2] = pass == 3 ? r.d : gxyz[2];
                 }
             }
         }
@@ -502,11 +530,9 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         // normal + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
         if (const uint64_t m = wave_ballot(want_normal)) {
             ev[E_NORMALS] += __builtin_popcountll(m);
-            V3<Dual> pd{Dual{L.px, 1.f, 0.f, 0.f}, Dual{L.py, 0.f, 1.f, 0.f}, Dual{L.pz, 0.f, 0.f, 1.f}};
-            Dual g = eval_lane_region<Dual>(nodes, regions, want_normal, use_region, pd);
             if (want_normal) {
-                float inv = f_rsqrt(g.x * g.x + g.y * g.y + g.z * g.z);
-                float nx = g.x * inv, ny = g.y * inv, nz = g.z * inv;
+                float inv = f_rsqrt(gxyz[0] * gxyz[0] + gxyz[1] * gxyz[1] + gxyz[2] * gxyz[2]);
+                float nx = gxyz[0] * inv, ny = gxyz[1] * inv, nz = gxyz[2] * inv;
                 const int nl = fn;
                 const float n1 = mt[L.layer].n, n2 = mt[nl].n;
                 float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);
@@ -528,11 +554,11 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 L.iface = true;
                 if (!rflag) {
                     L.layer = nl;
-                    trace_event(P, L, 3);
+                    trace_event(C, L, 3);
                 } else {
                     L.px = L.opx; L.py = L.opy; L.pz = L.opz;  // step back inside (:174)
-                    trace_event(P, L, 4);
-                    if (++L.bounces > P.max_bounces) end_kind = END_BOUNCE;
+                    trace_event(C, L, 4);
+                    if (++L.bounces > C->max_bounces) end_kind = END_BOUNCE;
                 }
                 if (end_kind < 0) L.st = ST_START;
             }
@@ -562,7 +588,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     if (len > 0.f) {
                         ev[E_CROSS]++;
                         if (ADJ) {
-                            const float w = P.weights ? P.weights[W.idx] : 1.f;
+                            const float w = C->weights ? C->weights[W.idx] : 1.f;
                             W.aint += adjC * len + 0.5f * w * len * len;
                             adjC += w * len;
                         }
@@ -624,7 +650,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             }
             L.px += d * L.dx; L.py += d * L.dy; L.pz += d * L.dz;
             if (!W.inside) {
-                end_kind = classify_escape(P, L);
+                end_kind = classify_escape(C, L);
             } else if (W.flight_over) {
                 // albedo test (src/kernelsMod.f90:287-297)
                 const DevMat m = mt[L.layer];
@@ -635,15 +661,15 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     }
                     L.nsc++;
                     out[O_SCATTERS]++;
-                    if ((P.flags & 1u) && L.nsc <= 4) {
-                        double* mo = P.moments + (L.nsc - 1) * 3;
+                    if ((C->flags & 1u) && L.nsc <= 4) {
+                        double* mo = C->moments + (L.nsc - 1) * 3;
                         atomicAdd(mo + 0, (double)L.px); atomicAdd(mo + 1, (double)L.py); atomicAdd(mo + 2, (double)L.pz);
                         atomicAdd(mo + 12, (double)L.px * L.px); atomicAdd(mo + 13, (double)L.py * L.py);
                         atomicAdd(mo + 14, (double)L.pz * L.pz);
                     }
                     scatter(P, L, m.hgg, m.g2);
-                    trace_event(P, L, 2);
-                    if ((P.flags & 2u) && L.nsc >= 5) {
+                    trace_event(C, L, 2);
+                    if ((C->flags & 2u) && L.nsc >= 5) {
                         end_kind = END_ABSORB;
                     } else {
                         L.tau = f_neglog(rnd(L, P));  // draw: next flight's optical depth
@@ -675,11 +701,11 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             pend_idx = -1;
             pend_li = -1;
             switch (end_kind) {
-            case END_ABSORB: out[O_ABSORBED]++; trace_event(P, L, 5); break;
-            case END_R: out[O_R]++; trace_event(P, L, 6); break;
-            case END_T: out[O_T]++; trace_event(P, L, 6); break;
-            case END_SURFACE: out[O_KSURF]++; trace_event(P, L, 7); break;
-            case END_BOUNCE: out[O_KBOUNCE]++; trace_event(P, L, 7); break;
+            case END_ABSORB: out[O_ABSORBED]++; trace_event(C, L, 5); break;
+            case END_R: out[O_R]++; trace_event(C, L, 6); break;
+            case END_T: out[O_T]++; trace_event(C, L, 6); break;
+            case END_SURFACE: out[O_KSURF]++; trace_event(C, L, 7); break;
+            case END_BOUNCE: out[O_KBOUNCE]++; trace_event(C, L, 7); break;
             default: break;  // error paths are counted where they occur
             }
             if (ADJ) {
@@ -688,7 +714,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 for (int m = 0; m < P.nmat; ++m) {
                     float Lm = adj_sc(A, 0, m, P.nmat), Am = adj_sc(A, 1, m, P.nmat);
                     float Km = adj_sc(A, 2, m, P.nmat), Bm = adj_sc(A, 3, m, P.nmat);
-                    float mus = P.mat_mus[m];
+                    float mus = C->mat_mus[m];
                     float scs = mus > 0.f ? Km / mus : 0.f;
                     if (isr) {
                         adj_g(A, 0, m, P.nmat) += (double)(-Lm);
@@ -720,8 +746,8 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
 
     // ---- flush counters ----
     for (int i = 0; i < NOUT; ++i)
-        if (out[i]) atomicAdd(P.stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
-    unsigned long long* evg = P.stats + (size_t)P.nsets * NOUT;
+        if (out[i]) atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
+    unsigned long long* evg = C->stats + (size_t)P.nsets * NOUT;
     for (int i = 0; i < NEV; ++i) {
         unsigned long long v = ev[i];
         const bool per_lane = (i == E_STEPS || i == E_CROSS || i == E_ATOMICS || i == E_NAN || i == E_UNREACH);
@@ -734,11 +760,11 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             for (int m = 0; m < P.nmat; ++m) {
                 double v = adj_g(A, w, m, P.nmat);
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane_id == 0) atomicAdd(P.grad + w * P.nmat + m, v);
+                if (lane_id == 0) atomicAdd(C->grad + w * P.nmat + m, v);
             }
         double v = adjW;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane_id == 0) atomicAdd(P.grad + 4 * P.nmat, v);
+        if (lane_id == 0) atomicAdd(C->grad + 4 * P.nmat, v);
     }
 }
 

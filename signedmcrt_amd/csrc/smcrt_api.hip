@@ -128,7 +128,7 @@ struct smcrt_scene {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
-        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum;
+        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold;
 };
 
 static int n_values_pushed(int op) { return op == SMCRT_OP_PRIM ? 1 : (op == SMCRT_OP_CSG ? -1 : 0); }
@@ -342,7 +342,7 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
         (void)hipSetDevice(s->device);
         DevBuf* bufs[] = {&s->d_nodes, &s->d_regions, &s->d_mats, &s->d_region_mat, &s->d_mat_mus, &s->d_work,
                           &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights,
-                          &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum};
+                          &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum, &s->d_cold};
         for (DevBuf* b : bufs) b->release();
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -405,8 +405,9 @@ static int ensure_device(smcrt_scene* s) {
     return SMCRT_OK;
 }
 
-static void fill_common(const smcrt_scene* s, KParams& P) {
+static void fill_common(const smcrt_scene* s, KParams& P, KCold& Cc) {
     memset(&P, 0, sizeof P);
+    memset(&Cc, 0, sizeof Cc);
     P.nodes = s->d_nodes.as<DevNode>();
     P.regions = s->d_regions.as<DevRegion>();
     P.mats = s->d_mats.as<DevMat>();
@@ -417,21 +418,21 @@ static void fill_common(const smcrt_scene* s, KParams& P) {
     P.sx = (float)(s->grid.nx / (2.0 * s->grid.xmax));
     P.sy = (float)(s->grid.ny / (2.0 * s->grid.ymax));
     P.sz = (float)(s->grid.nz / (2.0 * s->grid.zmax));
-    P.src_kind = s->src.kind;
-    P.circ_swap = s->circ_swap;
+    Cc.src_kind = s->src.kind;
+    Cc.circ_swap = s->circ_swap;
     for (int j = 0; j < 3; ++j) {
-        P.spos[j] = (float)s->src.pos[j];
-        P.sdir[j] = (float)s->src.dir[j];
-        P.p1[j] = (float)s->src.p1[j];
-        P.p2[j] = (float)s->src.p2[j];
-        P.p3[j] = (float)s->src.p3[j];
-        P.rdir[j] = s->rdir[j];
+        Cc.spos[j] = (float)s->src.pos[j];
+        Cc.sdir[j] = (float)s->src.dir[j];
+        Cc.p1[j] = (float)s->src.p1[j];
+        Cc.p2[j] = (float)s->src.p2[j];
+        Cc.p3[j] = (float)s->src.p3[j];
+        Cc.rdir[j] = s->rdir[j];
     }
-    P.radius = (float)s->src.radius;
-    memcpy(P.circ_t, s->circ_t, sizeof P.circ_t);
+    Cc.radius = (float)s->src.radius;
+    memcpy(Cc.circ_t, s->circ_t, sizeof Cc.circ_t);
     P.eps = s->eps;
     P.eps_kill = s->eps_kill;
-    P.mat_mus = s->d_mat_mus.as<float>();
+    Cc.mat_mus = s->d_mat_mus.as<float>();
 }
 
 struct LaunchShape {
@@ -548,7 +549,8 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
         if (rc) return rc;
     }
     KParams P;
-    fill_common(s, P);
+    KCold Cc;
+    fill_common(s, P, Cc);
     P.jmean = d_jmean;
     P.grid_stride = 0;
     P.seed_lo = (uint32_t)o->seed;
@@ -557,22 +559,22 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     P.nsets = 1;
     unsigned long long info[4] = {0ull, o->n_packets, o->first_packet, 0ull};
     HIPCHECK(hipMemcpyAsync(s->d_setinfo.p, info, sizeof info, hipMemcpyHostToDevice, stream));
-    P.set_prefix = s->d_setinfo.as<unsigned long long>();
-    P.set_first = s->d_setinfo.as<unsigned long long>() + 2;
+    Cc.set_prefix = s->d_setinfo.as<unsigned long long>();
+    Cc.set_first = s->d_setinfo.as<unsigned long long>() + 2;
     P.work = s->d_work.as<unsigned long long>();
-    P.stats = s->d_stats.as<unsigned long long>();
-    P.moments = s->d_moments.as<double>();
+    Cc.stats = s->d_stats.as<unsigned long long>();
+    Cc.moments = s->d_moments.as<double>();
     if (o->trace_packets > 0 && o->trace_events > 0) {
         size_t tb = (size_t)o->trace_packets * o->trace_events * 4 * sizeof(float);
         int rc = s->d_trace.ensure(tb);
         if (rc) return rc;
         HIPCHECK(hipMemsetAsync(s->d_trace.p, 0, tb, stream));
-        P.trace = s->d_trace.as<float>();
-        P.trace_packets = o->trace_packets;
-        P.trace_events = o->trace_events;
+        Cc.trace = s->d_trace.as<float>();
+        Cc.trace_packets = o->trace_packets;
+        Cc.trace_events = o->trace_events;
     }
-    P.flags = o->flags;
-    P.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
+    Cc.flags = o->flags;
+    Cc.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
     int tile_vox = 0;
     if (use_tile) {
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[a]; P.tn[a] = s->tile_tn[a]; }
@@ -581,12 +583,18 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox);
     P.chunk = sh.chunk;
     P.dda_cap = sh.walk_cap;
-    P.weights = d_w;
-    P.grad = s->d_grad.as<double>();
+    Cc.weights = d_w;
+    Cc.grad = s->d_grad.as<double>();
+    {
+        int rc = s->d_cold.ensure(sizeof(KCold));
+        if (rc) return rc;
+        HIPCHECK(hipMemcpyAsync(s->d_cold.p, &Cc, sizeof Cc, hipMemcpyHostToDevice, stream));
+        P.cold = s->d_cold.as<KCold>();
+    }
     k_fill_u64<<<1, 64, 0, stream>>>(P.work, 8);
-    k_fill_u64<<<1, 64, 0, stream>>>(P.stats, NOUT + NEV);
-    k_fill_f64<<<1, 64, 0, stream>>>(P.moments, 24);
-    if (adj) k_fill_f64<<<1, 256, 0, stream>>>(P.grad, 4 * s->mats.size() + 1);
+    k_fill_u64<<<1, 64, 0, stream>>>(Cc.stats, NOUT + NEV);
+    k_fill_f64<<<1, 64, 0, stream>>>(Cc.moments, 24);
+    if (adj) k_fill_f64<<<1, 256, 0, stream>>>(Cc.grad, 4 * s->mats.size() + 1);
     HIPCHECK(hipEventRecord(s->ev0, stream));
     if (adj) hipLaunchKernelGGL(packet_kernel<true>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
     else hipLaunchKernelGGL(packet_kernel<false>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
@@ -761,6 +769,7 @@ extern "C" int smcrt_grad(smcrt_scene* s, const smcrt_run_opts* o, const float* 
 // ------------------------------------------------------------------------------------------
 struct BatchGroup {
     smcrt_scene* scene;
+    KCold Cc;
     std::vector<int> sets;  // batch set indices in this group
     KParams P;
     LaunchShape sh;
@@ -778,7 +787,7 @@ struct smcrt_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    DevBuf d_mats, d_grids, d_stats, d_info, d_work;
+    DevBuf d_mats, d_grids, d_stats, d_info, d_work, d_cold;
     size_t total_doubles = 0, total_stats = 0;
 };
 
@@ -870,7 +879,7 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         info[io + g.sets.size()] = acc;
         io += 2 * g.sets.size() + 1;
         KParams& P = g.P;
-        fill_common(s, P);
+        fill_common(s, P, g.Cc);
         P.mats = b->d_mats.as<DevMat>() + mat_off;
         P.jmean = b->d_grids.as<double>() + g.grid_off;
         P.grid_stride = (unsigned long long)s->grid.nx * s->grid.ny * s->grid.nz;
@@ -878,12 +887,12 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         P.seed_hi = (uint32_t)(seed >> 32);
         P.total = acc;
         P.nsets = (int)g.sets.size();
-        P.set_prefix = b->d_info.as<unsigned long long>() + info_off;
-        P.set_first = b->d_info.as<unsigned long long>() + info_off + g.sets.size() + 1;
+        g.Cc.set_prefix = b->d_info.as<unsigned long long>() + info_off;
+        g.Cc.set_first = b->d_info.as<unsigned long long>() + info_off + g.sets.size() + 1;
         P.work = b->d_work.as<unsigned long long>() + gi;
-        P.stats = b->d_stats.as<unsigned long long>() + g.stats_off;
-        P.moments = s->d_moments.as<double>();
-        P.max_bounces = 1000;
+        g.Cc.stats = b->d_stats.as<unsigned long long>() + g.stats_off;
+        g.Cc.moments = s->d_moments.as<double>();
+        g.Cc.max_bounces = 1000;
         if (g.sets.size() * s->regions.size() > SMCRT_LDS_MATS) {
             smcrt_batch_destroy(b);
             return fail(SMCRT_ERANGE, "a geometry group holds %zu sets x %zu regions; at most %d material entries fit in LDS",
@@ -892,6 +901,14 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         g.sh = pick_shape(s, nullptr, false, (int)g.sets.size());
         P.chunk = g.sh.chunk;
         P.dda_cap = g.sh.walk_cap;
+    }
+    if ((rc = b->d_cold.ensure(b->groups.size() * sizeof(KCold)))) {
+        smcrt_batch_destroy(b);
+        return rc;
+    }
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        HIPCHECK(hipMemcpy(b->d_cold.as<KCold>() + gi, &b->groups[gi].Cc, sizeof(KCold), hipMemcpyHostToDevice));
+        b->groups[gi].P.cold = b->d_cold.as<KCold>() + gi;
     }
     HIPCHECK(hipMemcpy(b->d_mats.p, hm.data(), nmats * sizeof(DevMat), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(b->d_info.p, info.data(), ninfo * sizeof(unsigned long long), hipMemcpyHostToDevice));
@@ -962,6 +979,7 @@ extern "C" int smcrt_batch_destroy(smcrt_batch* b) {
     b->d_stats.release();
     b->d_info.release();
     b->d_work.release();
+    b->d_cold.release();
     delete b;
     return SMCRT_OK;
 }

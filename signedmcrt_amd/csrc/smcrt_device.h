@@ -99,7 +99,42 @@ SMCRT_DEV Dual d_max(const Dual& a, const Dual& b) { return b.v > a.v ? b : a; }
 SMCRT_DEV Dual d_sin(const Dual& a) { float s = sinf(a.v), c = cosf(a.v); return Dual{s, c * a.x, c * a.y, c * a.z}; }
 SMCRT_DEV Dual d_cos(const Dual& a) { float s = sinf(a.v), c = cosf(a.v); return Dual{c, -s * a.x, -s * a.y, -s * a.z}; }
 
-// generic wrappers so one template body serves float and Dual
+// D1: value and one directional derivative.  Three passes (seed x, y, z) give the gradient at a
+// third of Dual's register footprint; the packet kernel uses it for interface normals.
+struct D1 {
+    float v, d;
+};
+SMCRT_DEV float val(const D1& a) { return a.v; }
+SMCRT_DEV D1 operator+(const D1& a, const D1& b) { return D1{a.v + b.v, a.d + b.d}; }
+SMCRT_DEV D1 operator-(const D1& a, const D1& b) { return D1{a.v - b.v, a.d - b.d}; }
+SMCRT_DEV D1 operator-(const D1& a) { return D1{-a.v, -a.d}; }
+SMCRT_DEV D1 operator+(const D1& a, float b) { return D1{a.v + b, a.d}; }
+SMCRT_DEV D1 operator-(const D1& a, float b) { return D1{a.v - b, a.d}; }
+SMCRT_DEV D1 operator-(float b, const D1& a) { return D1{b - a.v, -a.d}; }
+SMCRT_DEV D1 operator*(const D1& a, float s) { return D1{a.v * s, a.d * s}; }
+SMCRT_DEV D1 operator*(float s, const D1& a) { return D1{a.v * s, a.d * s}; }
+SMCRT_DEV D1 operator*(const D1& a, const D1& b) { return D1{a.v * b.v, a.d * b.v + a.v * b.d}; }
+SMCRT_DEV D1 operator/(const D1& a, const D1& b) {
+    float r = 1.0f / b.v, q = a.v * r;
+    return D1{q, (a.d - q * b.d) * r};
+}
+SMCRT_DEV D1 operator/(const D1& a, float b) { return a * (1.0f / b); }
+
+// generic wrappers so one template body serves float, Dual and D1
+SMCRT_DEV D1 g_sqrt(const D1& a) {
+    float s = sqrtf(a.v);
+    return D1{s, s > 0.f ? a.d * (0.5f / s) : 0.f};
+}
+SMCRT_DEV D1 g_abs(const D1& a) { return a.v < 0.f ? -a : a; }
+SMCRT_DEV D1 g_min(const D1& a, const D1& b) { return b.v < a.v ? b : a; }
+SMCRT_DEV D1 g_max(const D1& a, const D1& b) { return b.v > a.v ? b : a; }
+SMCRT_DEV D1 g_max0(const D1& a) { return a.v > 0.f ? a : D1{0.f, 0.f}; }
+SMCRT_DEV D1 g_min0(const D1& a) { return a.v < 0.f ? a : D1{0.f, 0.f}; }
+SMCRT_DEV D1 g_clamp01(const D1& a) { return a.v < 0.f ? D1{0.f, 0.f} : (a.v > 1.f ? D1{1.f, 0.f} : a); }
+SMCRT_DEV D1 g_div(const D1& a, const D1& b) { return a / b; }
+SMCRT_DEV D1 g_sin(const D1& a) { return D1{sinf(a.v), cosf(a.v) * a.d}; }
+SMCRT_DEV D1 g_cos(const D1& a) { return D1{cosf(a.v), -sinf(a.v) * a.d}; }
+SMCRT_DEV D1 g_neg_if(const D1& a, bool neg) { return neg ? -a : a; }
 SMCRT_DEV float g_sqrt(float a) { return f_sqrt(a); }
 SMCRT_DEV Dual g_sqrt(const Dual& a) { return d_sqrt(a); }
 SMCRT_DEV float g_abs(float a) { return fabsf(a); }
@@ -123,6 +158,7 @@ SMCRT_DEV Dual g_cos(const Dual& a) { return d_cos(a); }
 template <class T> SMCRT_DEV T g_const(float c);
 template <> SMCRT_DEV float g_const<float>(float c) { return c; }
 template <> SMCRT_DEV Dual g_const<Dual>(float c) { return dconst(c); }
+template <> SMCRT_DEV D1 g_const<D1>(float c) { return D1{c, 0.f}; }
 // sign(s) * a with s's sign as a float multiplier
 SMCRT_DEV float g_neg_if(float a, bool neg) { return neg ? -a : a; }
 SMCRT_DEV Dual g_neg_if(const Dual& a, bool neg) { return neg ? -a : a; }

@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
-"""Throughput of the MI355X packet loop on the refracting-sphere workload.
+"""Throughput of the MI355X packet loop on the skin + vessels workload, and the adjoint's cost.
 
-A step is one persistent launch of the packet kernel over 1e7 packets of the refracting-sphere
-scene (clear sphere r = 0.5 cm, n = 1.5, in a 2 cm scattering cube, circular beam r = 0.3 cm),
-tallying the path-length fluence into a device-resident 200^3 FP64 grid.  Inputs (scene
-program, material table, grid) are resident in HBM before timing starts.
+A step is one persistent launch of the packet kernel over 1e8 packets of the skin + vessels scene
+(five stacked tissue boxes plus eight blood-vessel cylinders merged by SmoothUnion, in air;
+pencil beam), tallying the path-length fluence into a device-resident 256^3 FP64 grid.  Inputs
+(scene program, material table, grid) are resident in HBM before timing starts; the launch shape
+comes from the project's autotuner (tuned before timing, reported).
 
 Prints one JSON line: packets/s, the roofline of the packet kernel (FP32 VALU rate from the
 kernel's own event counters against the MI355X FP32 peak, kernel time from HIP events on the
-kernel's stream), the FP64 tally's byte rate against HBM bandwidth, and the CPU baseline: the
-FP64 C restatement of the reference loop (oracle/) timed on this host's cores on a bounded
-sample of the same workload, with the per-packet fluence error of both sides so throughput is
-compared at equal fluence error.
+kernel's stream), the FP64 tally's byte rate against HBM bandwidth, the CPU baseline (the FP64 C
+restatement of the reference loop, oracle/, timed on this host's cores on a bounded sample of the
+same workload, with the per-packet fluence error of both sides so throughput is compared at equal
+fluence error), and the gradient cost: one adjoint launch giving d(R_d, fluence sum)/d(mu_a, mu_s)
+of every material against the 2P forward launches of central finite differences.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload slab|refracting_sphere|skin_vessels]
 """
 from __future__ import annotations
 
@@ -30,6 +32,18 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (published)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E (published)
 METRIC = "photon packets/s @ fixed fluence error (1 GPU vs host CPU); grad cost vs FD"
+# workload -> (packets per step, description, FD parameters: materials whose (mu_a, mu_s) the
+# finite-difference comparison perturbs)
+WORKLOADS = {
+    "slab": (1_000_000, "slab 1x1x0.2 cm box mus=100 mua=0.1 g=0.9 n=1.37 in air, pencil beam, "
+             "100^3 grid", None),
+    "refracting_sphere": (10_000_000, "clear sphere r=0.5 n=1.5 mua=0.01 in 2 cm cube mus=50 "
+                          "mua=0.05 g=0.8 n=1.33, air, circular beam r=0.3 cm, 200^3 grid", None),
+    "skin_vessels": (100_000_000, "5 stacked tissue boxes (0.01-0.5 cm, mua 0.02-5, mus 20-300, "
+                     "g 0.8-0.95, n 1.37-1.45, seeded) + 8 blood cylinders r=0.01-0.05 mua=20 "
+                     "merged by SmoothUnion k=0.01, air, pencil beam, 256^3 grid",
+                     (1, 2, 3, 4, 5)),
+}
 
 
 def per_packet_rel_sd(batches):
@@ -58,8 +72,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--packets", type=int, default=10_000_000)
-    ap.add_argument("--workload", default="refracting_sphere")
+    ap.add_argument("--packets", type=int, default=0, help="packets per step (0: the workload's)")
+    ap.add_argument("--workload", default="skin_vessels", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-tune", action="store_true", help="library default launch shape")
+    ap.add_argument("--shape", default="", help="launch shape as JSON (skips the autotuner), e.g. "
+                    "'{\"block\": 256, \"chunk\": 256, \"walk_cap\": 8, \"refill_min\": 1, \"tile\": true}'")
+    ap.add_argument("--grad-steps", type=int, default=2,
+                    help="adjoint launches over the first timed step's packets (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-packets", type=int, default=2_000_000)
     ap.add_argument("--error-batches", type=int, default=10)
@@ -89,12 +108,24 @@ def main():
     from signedmcrt_amd import roofline, scenes
     from signedmcrt_amd.engine import Engine
 
+    n_default, description, fd_mats = WORKLOADS[args.workload]
     scene, _ = scenes.build(args.workload)
     g = scene.grid
     nv = g.nxg * g.nyg * g.nzg
     eng = Engine(scene)
     d_grid = torch.zeros(nv, dtype=torch.float64, device="cuda")
-    n = args.packets
+    n = args.packets or n_default
+
+    tuned = None
+    if args.shape:
+        eng.shape = json.loads(args.shape)
+        tuned = {"shape": eng.shape, "given": True}
+    elif not args.no_tune:
+        t_tune = time.perf_counter()
+        res = eng.tune(use_cache=False)
+        tuned = {"shape": res.shape, "probe_kernel_ms": res.kernel_ms,
+                 "default_probe_kernel_ms": res.default_ms, "speedup_vs_default": res.speedup,
+                 "points": len(res.table), "seconds": time.perf_counter() - t_tune}
 
     for w in range(args.warmup):
         eng.run_device(n, d_grid.data_ptr(), seed=1000 + w, first_packet=0)
@@ -132,6 +163,29 @@ def main():
         sd_gpu = per_packet_rel_sd(gb)
         del gb
 
+    # gradient cost: the adjoint launch over exactly the packets of the first timed step (same
+    # seed and ids: the same histories) against that step's forward launch.  Central finite
+    # differences need 2 forward launches per parameter.  (Comparing equal packet sets matters:
+    # a launch lasts at least as long as its longest history, ~1e5 scatters in a low-absorption
+    # layer, which for small launches can dominate the kernel time.)
+    grad = None
+    if fd_mats and args.grad_steps > 0:
+        ng = n
+        fwd_ms = stats[0]["kernel_ms"]
+        eng.grad(min(ng, 1_000_000), seed=77)  # warm (code load, LDS sizing)
+        gr = [eng.grad(ng, seed=123456789, first_packet=0) for _ in range(args.grad_steps)]
+        grad_ms = float(np.mean([x.stats["kernel_ms"] for x in gr]))
+        for k in ("launched", "absorbed", "escaped_r", "scatters"):
+            assert gr[0].stats[k] == stats[0][k], ("adjoint and forward histories differ", k)
+        p = 2 * len(fd_mats)
+        grad = {"packets": ng, "forward_kernel_ms": fwd_ms, "grad_kernel_ms": grad_ms,
+                "grad_cost_vs_forward": grad_ms / fwd_ms,
+                "fd_parameters": p, "fd_forward_launches": 2 * p,
+                "speedup_vs_central_fd": 2 * p * fwd_ms / grad_ms,
+                "derivatives_per_launch": 4 * scene.n_materials,
+                "d_rd_dmua": [float(v) for v in gr[0].d_r_dmua],
+                "d_rd_dmus": [float(v) for v in gr[0].d_r_dmus]}
+
     cpu = None
     if not args.no_cpu and B >= 2:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -168,9 +222,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "packets_per_step": n,
-                   "grid": [g.nxg, g.nyg, g.nzg], "tally": "fp64",
-                   "scene": "clear sphere r=0.5 n=1.5 mua=0.01 in 2 cm cube mus=50 mua=0.05 "
-                            "g=0.8 n=1.33, air, circular beam r=0.3 cm"},
+                   "grid": [g.nxg, g.nyg, g.nzg], "tally": "fp64", "scene": description,
+                   "launch_shape": tuned["shape"] if tuned else "library default"},
         "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP32_PEAK_TFLOPS,
                      "traffic": traffic, "kernel": "packet_kernel<false>",
@@ -186,6 +239,8 @@ def main():
                       ("absorbed", "escaped_r", "escaped_t", "killed_surface")},
         "per_packet_rel_sd": sd_gpu,
         "cpu_baseline": cpu,
+        "grad": grad,
+        "autotune": tuned,
     }
     print(json.dumps(out), flush=True)
 

@@ -165,7 +165,9 @@ typedef struct smcrt_run_opts {
     int32_t trace_packets; /* record the first `trace_packets` packets' events (0 = off)    */
     int32_t trace_events;  /* events per traced packet                                       */
     int32_t walk_cap;      /* voxel chords a lane walks per kernel-loop iteration, 0 -> default */
-    int32_t reserved[4];
+    int32_t refill_min;    /* regeneration policy: a wavefront refills its idle lanes once at
+                              least this many are idle (1 = at once; 0 -> default)            */
+    int32_t reserved[3];
 } smcrt_run_opts;
 
 /* How a packet ended.  A+R+T+K = launched. */
@@ -274,9 +276,12 @@ typedef struct smcrt_batch smcrt_batch; /* opaque: device buffers + captured gra
 
 /* Build a batch of n_sets property sets over n_scenes geometries.  Every set gets its own
    FP64 grid (its scene's grid shape) and its own stats.  The launch is captured once into a
-   HIP graph and replayed by smcrt_batch_run. */
+   HIP graph and replayed by smcrt_batch_run.  `shape` (may be NULL) supplies the launch shape
+   of every geometry group: block_size, blocks_per_cu, chunk, walk_cap, refill_min (its other
+   fields are ignored). */
 int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt_batch_set* sets,
-                       int32_t n_sets, uint64_t seed, smcrt_batch** out);
+                       int32_t n_sets, uint64_t seed, const smcrt_run_opts* shape,
+                       smcrt_batch** out);
 /* One persistent launch over every set (graph replay).  Clears the grids first. */
 int smcrt_batch_run(smcrt_batch* batch);
 /* Copy set `set`'s grid (raw FP64) and stats back; jmean may be NULL. */

@@ -73,7 +73,7 @@ class RunOpts(C.Structure):
                 ("flags", C.c_uint32), ("max_bounces", C.c_int32), ("block_size", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
                 ("trace_packets", C.c_int32), ("trace_events", C.c_int32),
-                ("walk_cap", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("walk_cap", C.c_int32), ("refill_min", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 STAT_COUNTERS = ["launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",
@@ -153,7 +153,7 @@ def lib() -> C.CDLL:
                                      P(C.c_float), P(C.c_float)]),
         "smcrt_grad": (C.c_int, [C.c_void_p, P(RunOpts), P(C.c_float), P(C.c_double), P(GradOut)]),
         "smcrt_batch_create": (C.c_int, [P(C.c_void_p), C.c_int32, P(BatchSet), C.c_int32,
-                                         C.c_uint64, P(C.c_void_p)]),
+                                         C.c_uint64, P(RunOpts), P(C.c_void_p)]),
         "smcrt_batch_run": (C.c_int, [C.c_void_p]),
         "smcrt_batch_read": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_double), P(Stats)]),
         "smcrt_batch_destroy": (C.c_int, [C.c_void_p]),

@@ -1,0 +1,188 @@
+"""Launch-shape autotuner for the packet kernel (no reference counterpart: the reference runs
+OpenMP static chunks, src/kernelsMod.f90:239-269).
+
+Knobs (`SPACE`):
+  block       threads per block of the persistent grid (64..256; resident blocks follow from
+              the occupancy query)
+  chunk       packet ids a wavefront claims per refill; chunk/64 = packets per lane per claim
+  walk_cap    voxel chords a lane walks per kernel-loop iteration before it sweeps again
+  refill_min  regeneration policy: a wavefront refills its idle lanes once at least this many
+              are idle (1 = at once)
+  tile        LDS-privatised FP32 tally tile on/off
+
+Every probe launches the same packets (fixed seed and first packet id).  Each packet draws from
+its own (seed, id) random stream, so the transported work is identical for every shape: the
+timing difference belongs to the shape alone, and the counters are checked equal across probes.
+Search: coordinate descent from the library defaults (two sweeps), best-of-`reps` kernel time per
+point.  Results are cached in a JSON file keyed by scene fingerprint, device name and library ABI
+version, so a scene is tuned once per device.
+
+    shape = autotune.tune(engine)            # dict of run keywords
+    engine.run(n, **shape)                   # or Engine(scene, shape=shape)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import os
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
+
+from . import _abi as A
+
+SPACE: Dict[str, Tuple] = {
+    "block": (64, 128, 256),
+    "chunk": (64, 128, 256, 512),
+    "walk_cap": (1, 2, 4, 8, 16),
+    "refill_min": (1, 8, 16, 32),
+    "tile": (True, False),
+}
+DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True}
+# counters that must not change with the launch shape
+INVARIANT = ("launched", "absorbed", "escaped_r", "escaped_t", "killed_surface", "killed_bounce",
+             "scatters", "ev_steps", "ev_crossings", "ev_interfaces")
+
+
+def run_kwargs(shape: Dict) -> Dict:
+    """Engine.run / run_device keywords for a shape dict."""
+    kw = {k: int(shape[k]) for k in ("block", "chunk", "walk_cap", "refill_min") if k in shape}
+    kw["flags"] = 0 if shape.get("tile", True) else A.FLAG_NO_LDS_TILE
+    return kw
+
+
+def coordinate_descent(cost: Callable[[Dict], float], space: Dict[str, Sequence],
+                       start: Dict, sweeps: int = 2) -> Tuple[Dict, List[Tuple[Dict, float]]]:
+    """Minimise `cost` over the product space one knob at a time, `sweeps` passes over the
+    knobs; each point is measured once (memoised).  Returns (best shape, [(shape, cost)])."""
+    memo: Dict[Tuple, float] = {}
+    table: List[Tuple[Dict, float]] = []
+
+    def f(sh: Dict) -> float:
+        key = tuple(sorted(sh.items()))
+        if key not in memo:
+            memo[key] = float(cost(dict(sh)))
+            table.append((dict(sh), memo[key]))
+        return memo[key]
+
+    best = dict(start)
+    best_c = f(best)
+    for _ in range(sweeps):
+        changed = False
+        for knob, values in space.items():
+            for v in values:
+                if v == best[knob]:
+                    continue
+                cand = dict(best)
+                cand[knob] = v
+                c = f(cand)
+                if c < best_c:
+                    best, best_c, changed = cand, c, True
+        if not changed:
+            break
+    return best, table
+
+
+def fingerprint(scene) -> str:
+    """Hash of everything the packet loop reads from the scene description."""
+    d = scene.desc
+    h = hashlib.sha1()
+
+    def put(ptr, n, ctype):
+        if n and ptr:
+            h.update(C.string_at(ptr, n * C.sizeof(ctype)))
+
+    put(d.nodes, d.n_nodes, A.Node)
+    put(d.regions, d.n_regions, A.Region)
+    put(d.materials, d.n_materials, A.Material)
+    h.update(bytes(d.grid))
+    h.update(bytes(d.source))
+    return h.hexdigest()[:20]
+
+
+def default_cache_path() -> str:
+    base = os.environ.get("SMCRT_CACHE_DIR") or os.path.join(os.path.expanduser("~"), ".cache", "smcrt")
+    return os.path.join(base, "autotune.json")
+
+
+@dataclass
+class TuneResult:
+    shape: Dict
+    kernel_ms: float           # best-of-reps kernel time of the probe at `shape`
+    default_ms: float          # same at the library defaults
+    table: List[Tuple[Dict, float]] = field(default_factory=list)
+    cached: bool = False
+
+    @property
+    def speedup(self) -> float:
+        return self.default_ms / self.kernel_ms if self.kernel_ms > 0 else 1.0
+
+
+def _device_name() -> str:
+    import torch
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return f"{p.name}:{getattr(p, 'gcnArchName', '')}:{p.multi_processor_count}"
+
+
+def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
+         cache_path: Optional[str] = None, use_cache: bool = True,
+         space: Optional[Dict[str, Sequence]] = None, seed: int = 7919) -> TuneResult:
+    """Tune the launch shape of `engine`'s scene on the current GPU.
+
+    n_probe: packets per probe (default: enough for ~30 ms of kernel time at the default shape,
+    estimated from a first probe).  The probe tallies into a scratch device grid."""
+    import torch
+
+    space = dict(space or SPACE)
+    cache_path = cache_path or default_cache_path()
+    key = f"{fingerprint(engine.scene)}|{_device_name()}|abi{A.ABI_VERSION}"
+    if use_cache and os.path.exists(cache_path):
+        try:
+            entry = json.load(open(cache_path)).get(key)
+        except (OSError, ValueError):
+            entry = None
+        if entry:
+            return TuneResult(entry["shape"], entry["kernel_ms"], entry["default_ms"], [], cached=True)
+
+    g = engine.scene.grid
+    grid = torch.zeros(g.nxg * g.nyg * g.nzg, dtype=torch.float64, device="cuda")
+    ref: Dict = {}
+
+    def probe(shape: Dict, n: int) -> float:
+        best = float("inf")
+        for _ in range(reps):
+            st = engine.run_device(n, grid.data_ptr(), seed=seed, first_packet=1 << 50,
+                                   **run_kwargs(shape))
+            inv = {k: st[k] for k in INVARIANT}
+            if n in ref and ref[n] != inv:
+                raise RuntimeError(f"launch shape {shape} changed the transported work: "
+                                   f"{inv} != {ref[n]}")
+            ref.setdefault(n, inv)
+            best = min(best, st["kernel_ms"])
+        return best
+
+    start = {k: DEFAULT[k] for k in space}
+    if n_probe is None:
+        n0 = 200_000
+        probe(start, n0)  # warm-up: the LDS tile's pilot run, code load
+        ms = probe(start, n0)
+        n_probe = int(min(max(n0 * 30.0 / max(ms, 1e-3), 200_000), 20_000_000))
+    t0 = time.perf_counter()
+    best, table = coordinate_descent(lambda sh: probe(sh, n_probe), space, start, sweeps)
+    best_ms = dict((tuple(sorted(s.items())), c) for s, c in table)[tuple(sorted(best.items()))]
+    default_ms = table[0][1]
+    res = TuneResult(best, best_ms, default_ms, table)
+    if use_cache:
+        try:
+            os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+            data = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
+        except (OSError, ValueError):
+            data = {}
+        data[key] = {"shape": best, "kernel_ms": best_ms, "default_ms": default_ms,
+                     "n_probe": n_probe, "seconds": time.perf_counter() - t0}
+        tmp = cache_path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache_path)
+    return res

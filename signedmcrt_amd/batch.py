@@ -1,6 +1,6 @@
 """Batched property sets in one persistent launch under a HIP graph (no reference counterpart).
 
-    b = Batch([eng_slab, eng_sphere], [BatchSet(0, mats, 2_000_000), ...], seed)
+    b = Batch([eng_slab, eng_sphere], [BatchSet(0, mats, 2_000_000), ...], seed, **shape)
     b.run()                     # one graph replay: clears grids/counters, runs every set
     jmean, stats = b.read(i)    # set i's raw FP64 grid (its scene's grid shape) and counters
 
@@ -17,7 +17,7 @@ from typing import List, Sequence
 import numpy as np
 
 from . import _abi as A
-from .engine import Engine
+from .engine import Engine, run_opts
 
 
 @dataclass
@@ -30,7 +30,8 @@ class BatchSet:
 
 
 class Batch:
-    def __init__(self, engines: List[Engine], sets: List[BatchSet], seed: int = 123456789):
+    def __init__(self, engines: List[Engine], sets: List[BatchSet], seed: int = 123456789,
+                 **shape):
         self._lib = A.lib()
         self.engines = engines
         self.sets = sets
@@ -46,7 +47,10 @@ class Batch:
         self._sets = arr
         handles = (C.c_void_p * len(engines))(*[e.handle.value for e in engines])
         h = C.c_void_p()
-        A.check(self._lib.smcrt_batch_create(handles, len(engines), arr, len(sets), int(seed), C.byref(h)))
+        self.shape = dict(shape)
+        sh = run_opts(0, **shape) if shape else None
+        A.check(self._lib.smcrt_batch_create(handles, len(engines), arr, len(sets), int(seed),
+                                             C.byref(sh) if sh is not None else None, C.byref(h)))
         self.handle = h
 
     def run(self) -> None:

@@ -87,6 +87,7 @@ struct KParams {
     int nsets;
     unsigned long long* work;              // work counter
     int chunk, dda_cap;
+    int refill_min;  // regeneration policy: refill once at least this many lanes are idle
     // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP32 LDS per block
     // and are flushed into the FP64 grid at the end (tn[0] == 0: no tile)
     int t0[3], tn[3];
@@ -328,6 +329,8 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         // ---- regeneration: idle lanes take the next packet ids of the wavefront's chunk ----
         if (!exhausted) {
             uint64_t need = wave_ballot(L.st == ST_IDLE);
+            // regeneration policy: wait until enough lanes are idle (or the whole wave is)
+            if (need != ~0ull && __builtin_popcountll(need) < P.refill_min) need = 0;
             while (need) {
                 if (avail == 0) {
                     unsigned long long b = 0;
@@ -353,8 +356,10 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                         s = lo;
                     }
                     if (s != cur_set) {
+#pragma unroll
                         for (int i = 0; i < NOUT; ++i)
                             if (out[i]) atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
+#pragma unroll
                         for (int i = 0; i < NOUT; ++i) out[i] = 0;
                         cur_set = s;
                     }
@@ -745,9 +750,11 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     }
 
     // ---- flush counters ----
+#pragma unroll
     for (int i = 0; i < NOUT; ++i)
         if (out[i]) atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
     unsigned long long* evg = C->stats + (size_t)P.nsets * NOUT;
+#pragma unroll
     for (int i = 0; i < NEV; ++i) {
         unsigned long long v = ev[i];
         const bool per_lane = (i == E_STEPS || i == E_CROSS || i == E_ATOMICS || i == E_NAN || i == E_UNREACH);

@@ -436,7 +436,7 @@ static void fill_common(const smcrt_scene* s, KParams& P, KCold& Cc) {
 }
 
 struct LaunchShape {
-    int block, blocks, chunk, walk_cap;
+    int block, blocks, chunk, walk_cap, refill_min;
     size_t lds;
 };
 
@@ -448,6 +448,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
+    L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
     const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
     L.lds = mats;
     L.block = (L.block / 64) * 64;
@@ -583,6 +584,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox);
     P.chunk = sh.chunk;
     P.dda_cap = sh.walk_cap;
+    P.refill_min = sh.refill_min;
     Cc.weights = d_w;
     Cc.grad = s->d_grad.as<double>();
     {
@@ -792,7 +794,8 @@ struct smcrt_batch {
 };
 
 extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt_batch_set* sets,
-                                  int32_t n_sets, uint64_t seed, smcrt_batch** out) {
+                                  int32_t n_sets, uint64_t seed, const smcrt_run_opts* shape,
+                                  smcrt_batch** out) {
     if (!scenes || !sets || !out || n_scenes < 1 || n_sets < 1) return fail(SMCRT_EINVAL, "bad batch arguments");
     *out = nullptr;
     for (int i = 0; i < n_scenes; ++i) {
@@ -898,9 +901,10 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
             return fail(SMCRT_ERANGE, "a geometry group holds %zu sets x %zu regions; at most %d material entries fit in LDS",
                         g.sets.size(), s->regions.size(), SMCRT_LDS_MATS);
         }
-        g.sh = pick_shape(s, nullptr, false, (int)g.sets.size());
+        g.sh = pick_shape(s, shape, false, (int)g.sets.size());
         P.chunk = g.sh.chunk;
         P.dda_cap = g.sh.walk_cap;
+        P.refill_min = g.sh.refill_min;
     }
     if ((rc = b->d_cold.ensure(b->groups.size() * sizeof(KCold)))) {
         smcrt_batch_destroy(b);

@@ -56,17 +56,21 @@ def _fptr(a: np.ndarray):
 
 def run_opts(n: int, seed: int = 123456789, first_packet: int = 0, flags: int = 0,
              block: int = 0, blocks_per_cu: int = 0, chunk: int = 0, max_bounces: int = 0,
-             trace_packets: int = 0, trace_events: int = 0, walk_cap: int = 0) -> A.RunOpts:
+             trace_packets: int = 0, trace_events: int = 0, walk_cap: int = 0,
+             refill_min: int = 0) -> A.RunOpts:
     o = A.RunOpts()
     o.n_packets, o.seed, o.first_packet, o.flags = int(n), int(seed), int(first_packet), int(flags)
     o.block_size, o.blocks_per_cu, o.chunk, o.max_bounces = block, blocks_per_cu, chunk, max_bounces
     o.trace_packets, o.trace_events, o.walk_cap = trace_packets, trace_events, walk_cap
+    o.refill_min = refill_min
     return o
 
 
 class Engine:
-    def __init__(self, scene: Scene):
+    def __init__(self, scene: Scene, shape: Optional[dict] = None):
+        """`shape`: launch shape applied to every run (an autotune result, see `tune`)."""
         self.scene = scene
+        self.shape = dict(shape or {})
         self._lib = A.lib()
         h = C.c_void_p()
         A.check(self._lib.smcrt_scene_create(C.byref(scene.desc), C.byref(h)))
@@ -83,11 +87,30 @@ class Engine:
         except Exception:
             pass
 
+    def _shape(self, flags: int, shape: dict):
+        """Explicit keywords win over the engine's tuned shape; flags combine."""
+        if not self.shape:
+            return flags, shape
+        from .autotune import run_kwargs
+        kw = run_kwargs(self.shape)
+        flags |= kw.pop("flags")
+        kw.update(shape)
+        return flags, kw
+
+    def tune(self, **kw):
+        """Autotune the launch shape on the current GPU (cached per scene and device) and use it
+        for later runs.  Returns the autotune.TuneResult."""
+        from . import autotune
+        res = autotune.tune(self, **kw)
+        self.shape = dict(res.shape)
+        return res
+
     # -- the packet loop -------------------------------------------------------------------
     def run(self, n: int, seed: int = 123456789, first_packet: int = 0, flags: int = 0,
             tally: bool = True, trace_packets: int = 0, trace_events: int = 0,
             **shape) -> RunResult:
         g = self.scene.grid
+        flags, shape = self._shape(flags, shape)
         o = run_opts(n, seed, first_packet, flags, trace_packets=trace_packets,
                      trace_events=trace_events, **shape)
         t = A.Tallies()
@@ -107,6 +130,7 @@ class Engine:
                    flags: int = 0, stream: int = 0, **shape) -> dict:
         """Packet loop into a device-resident FP64 grid (a raw device pointer, e.g.
         torch.Tensor.data_ptr()); returns the counters."""
+        flags, shape = self._shape(flags, shape)
         o = run_opts(n, seed, first_packet, flags, **shape)
         st = A.Stats()
         A.check(self._lib.smcrt_run_device(self.handle, C.byref(o), C.c_void_p(d_jmean_ptr),
@@ -132,7 +156,8 @@ class Engine:
              tally: bool = False, **shape) -> GradResult:
         g = self.scene.grid
         nm = self.scene.n_materials
-        o = run_opts(n, seed, first_packet, **shape)
+        flags, shape = self._shape(0, shape)
+        o = run_opts(n, seed, first_packet, flags, **shape)
         out = A.GradOut()
         arrs = [np.zeros(nm) for _ in range(4)]
         out.d_r_dmua, out.d_r_dmus, out.d_w_dmua, out.d_w_dmus = (_dptr(a) for a in arrs)

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMCRT_ABI_VERSION 1
+#define SMCRT_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------------- */
 enum {
@@ -132,6 +132,34 @@ typedef struct smcrt_source {
     double radius;   /* circular                                                          */
 } smcrt_source;
 
+/* ---- detectors (src/detectors/detectors.f90, src/detectors/detector_base.f90) ------------
+ * Every packet, when it ends, is offered to every detector with the hit point the reference
+ * builds (src/kernelsMod.f90:302-306): its final position and direction, value = sqrt(x^2+y^2)
+ * of the final position, and its layer.  A detector counts it when `layer` matches and:
+ *  circle   the ray from the final position along the direction meets the detector plane
+ *           (normal `dir`, denominator > 1e-6) within `radius1` of `pos`, at t <= 5e-3
+ *           (check_hit_circle, intersectCircle/intersectPlane in src/geometryMod.f90:217-271)
+ *  annulus  radius1 <= |final position - pos| <= radius2 (check_hit_annulus)
+ *  camera   the ray meets the plane through p1 with normal unit((p3-p1) x (p2-p1)) at t >= 0
+ *           inside the p1/p2/p3 parallelogram (check_hit_camera)
+ * 1-D detectors (circle, annulus) have nbins+1 bins of width maxval/(nbins-1) (circle) or
+ * maxval/nbins (annulus) — 1 if nbins == 0 — and bin min(nint(value/width)+1, nbins+1);
+ * the camera has (nbins+1)^2 bins of width maxval/(nbins+1), indexed by
+ * (final z + p1.x, final y + p1.y) as record_hit_2D_sub does.  Counts are exact (64-bit). */
+#define SMCRT_MAX_DETECTORS 16
+enum { SMCRT_DET_CIRCLE = 1, SMCRT_DET_ANNULUS = 2, SMCRT_DET_CAMERA = 3 };
+
+typedef struct smcrt_detector {
+    int32_t kind;          /* SMCRT_DET_*                                                      */
+    int32_t layer;         /* the reference's 1-based layer (region index + 1)                 */
+    int32_t nbins;         /* as written in the input (the detector holds nbins+1 bins)        */
+    int32_t track_history; /* accepted and ignored (history stacks are outside this path)      */
+    double pos[3];         /* circle/annulus `position`; camera `p1`                          */
+    double dir[3];         /* circle/annulus `direction`                                       */
+    double p2[3], p3[3];   /* camera                                                           */
+    double radius1, radius2, maxval;
+} smcrt_detector;
+
 typedef struct smcrt_scene_desc {
     uint32_t abi_version; /* SMCRT_ABI_VERSION                                           */
     int32_t n_nodes, n_regions, n_materials;
@@ -140,6 +168,9 @@ typedef struct smcrt_scene_desc {
     const smcrt_material* materials;
     smcrt_grid grid;
     smcrt_source source;
+    int32_t n_detectors;  /* 0..SMCRT_MAX_DETECTORS                                       */
+    int32_t pad;
+    const smcrt_detector* detectors;
 } smcrt_scene_desc;
 
 /* ---- run options, statistics, tallies -------------------------------------------------- */
@@ -198,6 +229,8 @@ typedef struct smcrt_tallies {
     double* jmean;     /* nx*ny*nz FP64 raw path-length sums (may be NULL)                    */
     float* jmean_f32;  /* optional FP32 copy, the reference's `jmean` type (may be NULL)      */
     float* trace;      /* optional: trace_packets*trace_events*4 floats (x,y,z,event)         */
+    uint64_t* detector_counts; /* optional: every detector's bins, concatenated in detector
+                                  order (smcrt_detector_bins() each; may be NULL)              */
     smcrt_stats stats;
 } smcrt_tallies;
 
@@ -288,11 +321,60 @@ int smcrt_batch_run(smcrt_batch* batch);
 int smcrt_batch_read(smcrt_batch* batch, int32_t set, double* jmean, smcrt_stats* stats);
 int smcrt_batch_destroy(smcrt_batch* batch);
 
+/* Number of bins of detector `i` of a scene (0 if out of range). */
+int64_t smcrt_detector_bins(const smcrt_scene* scene, int32_t i);
+
+/* ---- input: the reference's TOML schema (src/parse/parse.f90, parse_source.f90,
+ * parse_detectors.f90) -------------------------------------------------------------------
+ * smcrt_config_load reads a TOML input file into a flat, caller-owned struct, applying the
+ * reference's defaults and its error checks (status SMCRT_EINVAL + message from
+ * smcrt_last_error; SMCRT_EIO if the file cannot be read).  Strings are NUL-terminated.
+ * Departure: a vector `direction` (e.g. [0.0, 0.0, -1.0]) is accepted together with the rest
+ * of the source table; the reference returns early there and leaves the packet unset
+ * (src/parse/parse_source.f90:99-108). */
+typedef struct smcrt_config {
+    /* [source] */
+    char source_name[32];     /* "point"                                                 */
+    int64_t nphotons;         /* 1000000                                                 */
+    int32_t has_position, has_direction;
+    double position[3], direction[3];
+    double point1[3], point2[3], point3[3]; /* (-1,-1,1), (2,0,0), (0,2,0)                */
+    double radius, beta, radius_hi;         /* 0.5, 5, 0.6                               */
+    char annulus_type[32];                  /* "gaussian"                                */
+    /* [grid] */
+    int32_t nxg, nyg, nzg, pad0;            /* 200                                       */
+    double xmax, ymax, zmax;                /* 1.0                                       */
+    char units[16];                         /* "cm"                                      */
+    /* [geometry] */
+    char geom_name[64];                     /* "sphere"                                  */
+    double tau, musb, muab, musc, muac, hgg;/* 10, 0, 0.01, 0, 0.01, 0.7                 */
+    int32_t num_spheres, pad1;              /* 10                                        */
+    /* [output] */
+    char fluence[256], absorb[256], render[256];  /* fluence.nrrd, absorb.nrrd, geom_render.nrrd */
+    int32_t render_geom, overwrite;
+    int32_t render_size[3], pad2;           /* 200 200 200                               */
+    /* [simulation] */
+    int64_t iseed;                          /* 123456789                                 */
+    int32_t tev, absorb_flag, load_checkpoint, checkpoint_every_n;
+    char checkpoint_file[256];
+    /* [[detectors]] */
+    int32_t n_detectors, pad3;
+    smcrt_detector detectors[SMCRT_MAX_DETECTORS];
+    char history_file[256];                 /* "photPos.obj"                             */
+} smcrt_config;
+
+int smcrt_config_load(const char* path, smcrt_config* out);
+
 /* ---- host-side output helpers (src/writer.f90) ----------------------------------------- */
 
 /* `normalise_fluence` (src/writer.f90:25-50): out = raw * (2x*2y)/(N*dx*dy*dz). */
 int smcrt_normalise_fluence(const smcrt_grid* grid, const double* raw, uint64_t n_packets,
                             float* out);
+
+/* write_detected_photons (src/writer.f90:53-84): file `path` gets, for a circle detector, one
+   line per bin j = 1..nbins+1 with j*bin_width and the count; annulus and camera detectors get
+   an empty file, as in the reference. */
+int smcrt_write_detector(const char* path, const smcrt_detector* det, const uint64_t* counts);
 
 /* Write a 3-D FP32 array as NRRD the way `write_3d_r4_nrrd` does (src/writer.f90:220-251,
    294-334): header, optional metadata lines (`meta`, may be NULL), blank line, raw data. */

@@ -39,7 +39,8 @@ class OrcOpts(C.Structure):
 class OrcOut(C.Structure):
     _fields_ = [("jmean", _dp), ("trace", _P(C.c_float)),
                 ("d_r_dmua", _dp), ("d_r_dmus", _dp), ("d_w_dmua", _dp), ("d_w_dmus", _dp),
-                ("w_tally", C.c_double), ("stats", A.Stats), ("seconds", C.c_double)]
+                ("w_tally", C.c_double), ("stats", A.Stats), ("seconds", C.c_double),
+                ("det_counts", _P(C.c_uint64))]
 
 
 _lib = None
@@ -63,6 +64,8 @@ def lib() -> C.CDLL:
         L.orc_emit.argtypes = [_P(A.SceneDesc), C.c_int64, C.c_uint64, C.c_uint64, _dp, _dp]
         L.orc_walk.restype = C.c_double
         L.orc_walk.argtypes = [_P(A.SceneDesc), _dp, _dp, C.c_double, _dp, _P(C.c_int)]
+        L.orc_detector_hits.restype = None
+        L.orc_detector_hits.argtypes = [_P(A.Detector), C.c_int64, _dp, _dp, _P(C.c_int), _P(C.c_uint64)]
         L.orc_uniforms.restype = None
         L.orc_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, _dp]
         _lib = L
@@ -120,6 +123,18 @@ def walk(scene, pos, dir, dstep: float):
     return t, jm.reshape(g.shape, order="F"), bool(flag.value)
 
 
+def detector_hits(det, pos, dirs, layers):
+    """Bins of detector `det` (signedmcrt_amd.detectors description) after the given hit points."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    lay = np.ascontiguousarray(np.broadcast_to(layers, (len(pos),)), dtype=np.int32)
+    counts = np.zeros(int(np.prod(det.shape)), dtype=np.uint64)
+    dc = det.to_c()
+    lib().orc_detector_hits(C.byref(dc), len(pos), _ptr(pos), _ptr(dirs), _ptr(lay, C.c_int),
+                            _ptr(counts, C.c_uint64))
+    return counts.reshape(det.shape, order="F")
+
+
 def uniforms(seed: int, pid: int, n: int):
     out = np.empty(n)
     lib().orc_uniforms(seed, pid, n, _ptr(out))
@@ -152,11 +167,21 @@ def run(scene, n_packets: int, seed: int = 123456789, first_packet: int = 0, fla
     gr = [np.zeros(nm) for _ in range(4)]
     if grad:
         out.d_r_dmua, out.d_r_dmus, out.d_w_dmua, out.d_w_dmus = (_ptr(x) for x in gr)
+    dets = getattr(scene, "detectors", [])
+    sizes = [int(np.prod(dd.shape)) for dd in dets]
+    counts = np.zeros(max(1, sum(sizes)), dtype=np.uint64)
+    if dets:
+        out.det_counts = _ptr(counts, C.c_uint64)
     rc = lib().orc_run(C.byref(scene.desc), C.byref(o), C.byref(out))
     if rc != 0:
         raise RuntimeError(f"orc_run failed ({rc})")
     res = {"jmean": jm.reshape(g.shape, order="F") if tally else None,
            "stats": out.stats.as_dict(), "seconds": out.seconds}
+    if dets:
+        res["detectors"], off = [], 0
+        for dd, sz in zip(dets, sizes):
+            res["detectors"].append(counts[off:off + sz].reshape(dd.shape, order="F").copy())
+            off += sz
     if trace_packets:
         res["trace"] = tr.reshape(trace_packets, trace_events, 4)
     if grad:

@@ -650,7 +650,105 @@ typedef struct orc_out {
     double w_tally;
     smcrt_stats stats;
     double seconds;         /* wall time of the packet loop (omp_get_wtime) */
+    uint64_t* det_counts;   /* every detector's bins, concatenated (NULL: not recorded) */
 } orc_out;
+
+/* ---------------------------------------------------------------------------------------
+ * detectors: constructors (src/detectors/detectors.f90 init_circle_dect, init_annulus_dect,
+ * init_camera; the circle's direction normalised by src/parse/parse_detectors.f90), hit tests
+ * (check_hit_circle with intersectCircle/intersectPlane, src/geometryMod.f90:217-271;
+ * check_hit_annulus; check_hit_camera) and binning (record_hit_1D_sub / record_hit_2D_sub,
+ * src/detectors/detector_base.f90:106-162)
+ * ------------------------------------------------------------------------------------- */
+typedef struct {
+    int kind, layer, nb;  /* layer 1-based; nb = nbins + 1 bins (per axis for the camera) */
+    size_t off;
+    v3 pos, dir, e1, e2, n;
+    double radius, r1, r2, width, height, bw, bwx, bwy;
+} orc_det;
+
+static size_t det_setup(const smcrt_detector* in, orc_det* d) {
+    memset(d, 0, sizeof *d);
+    d->kind = in->kind;
+    d->layer = in->layer;
+    d->nb = in->nbins + 1;
+    d->pos = mk(in->pos[0], in->pos[1], in->pos[2]);
+    d->dir = mk(in->dir[0], in->dir[1], in->dir[2]);
+    if (in->kind == SMCRT_DET_CIRCLE) {
+        d->dir = norm(d->dir); /* dir%magnitude() */
+        d->radius = in->radius1;
+        d->bw = in->nbins == 0 ? 1.0 : in->maxval / (double)(in->nbins - 1);
+        return (size_t)d->nb;
+    }
+    if (in->kind == SMCRT_DET_ANNULUS) {
+        d->r1 = in->radius1;
+        d->r2 = in->radius2;
+        d->bw = in->nbins == 0 ? 1.0 : in->maxval / (double)in->nbins;
+        return (size_t)d->nb;
+    }
+    v3 p2 = mk(in->p2[0], in->p2[1], in->p2[2]), p3 = mk(in->p3[0], in->p3[1], in->p3[2]);
+    d->e1 = sub(p2, d->pos);
+    d->e2 = sub(p3, d->pos);
+    d->width = sqrt(dot(d->e1, d->e1));
+    d->height = sqrt(dot(d->e2, d->e2));
+    v3 c = mk(d->e2.y * d->e1.z - d->e2.z * d->e1.y, d->e2.z * d->e1.x - d->e2.x * d->e1.z,
+              d->e2.x * d->e1.y - d->e2.y * d->e1.x); /* e2 .cross. e1 */
+    d->n = norm(c);
+    d->bwx = in->nbins == 0 ? 1.0 : in->maxval / (double)d->nb;
+    d->bwy = d->bwx;
+    return (size_t)d->nb * (size_t)d->nb;
+}
+
+/* nint(q) + 1 clamped to nb, as a 0-based bin; -1 where the reference's index would leave the
+   array (q negative or NaN) */
+static long bin1d(double q, int nb) {
+    if (!(q >= 0.0)) return -1;
+    long i = q < 9.0e18 ? (long)llround(q) + 1 : nb;
+    return (i < nb ? i : nb) - 1;
+}
+
+/* int(q) + 1 clamped to nb, wrapped to nb below 1 (record_hit_2D_sub); 0 for NaN */
+static long axis_bin(double q, int nb) {
+    if (q != q) return 0;
+    long i = q >= 9.0e18 ? nb : (q <= -9.0e18 ? 0 : (long)q + 1);
+    if (i > nb) i = nb;
+    return i < 1 ? nb : i;
+}
+
+/* record one hit point (pos, dir, value = sqrt(x^2+y^2), layer 1-based) on detector d */
+static void det_record(const orc_det* d, v3 pos, v3 dir, int layer, uint64_t* counts) {
+    if (layer != d->layer) return;
+    double value = sqrt(pos.x * pos.x + pos.y * pos.y);
+    long idx = -1; /* 0-based bin */
+    if (d->kind == SMCRT_DET_CIRCLE) {
+        double denom = dot(d->dir, dir);
+        if (!(denom > 1e-6)) return;
+        double t = dot(sub(d->pos, pos), d->dir) / denom;
+        if (!(t >= 0.0)) return;
+        v3 v = sub(add(pos, scl(dir, t)), d->pos);
+        if (!(sqrt(dot(v, v)) <= d->radius)) return;
+        if (t > 5e-3) return;
+        idx = bin1d(value / d->bw, d->nb);
+    } else if (d->kind == SMCRT_DET_ANNULUS) {
+        v3 v = sub(pos, d->pos);
+        double r = sqrt(dot(v, v));
+        if (!(r >= d->r1 && r <= d->r2)) return;
+        idx = bin1d(value / d->bw, d->nb);
+    } else {
+        double t = dot(sub(d->pos, pos), d->n) / dot(dir, d->n);
+        if (!(t >= 0.0)) return;
+        v3 v = sub(add(pos, scl(dir, t)), d->pos);
+        double q1 = dot(v, d->e1) / d->width, q2 = dot(v, d->e2) / d->height;
+        if (!(q1 < d->width && q1 > 0.0 && q2 < d->height && q2 > 0.0)) return;
+        double x = pos.z + d->pos.x, y = pos.y + d->pos.y; /* as record_hit_2D_sub */
+        long ix = axis_bin(x / d->bwx, d->nb), iy = axis_bin(y / d->bwy, d->nb);
+        if (ix < 1 || iy < 1) return;
+        idx = (ix - 1) + (long)d->nb * (iy - 1);
+    }
+    if (idx < 0) return;
+#pragma omp atomic
+    counts[d->off + (size_t)idx] += 1;
+}
 
 static void trace_put(float* tr, int ne, int* cnt, v3 p, int ev) {
     if (!tr || *cnt >= ne) return;
@@ -681,6 +779,17 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
     uint64_t e_sw = 0, e_re = 0, e_no = 0, e_st = 0, e_cr = 0, e_if = 0, e_fr = 0, e_un = 0, e_dc = 0;
     double wt = 0.0;
     int64_t N = (int64_t)o->n_packets;
+    orc_det dets[SMCRT_MAX_DETECTORS];
+    int ndet = out->det_counts ? d->n_detectors : 0;
+    {
+        size_t off = 0;
+        for (int i = 0; i < ndet; ++i) {
+            size_t nb = det_setup(&d->detectors[i], &dets[i]);
+            dets[i].off = off;
+            for (size_t b = 0; b < nb; ++b) out->det_counts[off + b] = 0;
+            off += nb;
+        }
+    }
 #pragma omp parallel reduction(+ : a_abs, a_r, a_t, a_ks, a_kb, a_sc, a_err, e_sw, e_re, e_no, e_st, e_cr, e_if, e_fr, e_un, e_dc, wt)
     {
         orc_thread T;
@@ -751,6 +860,9 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
             case END_BOUNCE: a_kb++; trace_put(tr, o->trace_events, &k.tc, k.pos, 7); break;
             default: a_err++; break;
             }
+            /* detectors see every ended history (src/kernelsMod.f90:302-306) */
+            if (k.reason != END_ERROR)
+                for (int i = 0; i < ndet; ++i) det_record(&dets[i], k.pos, k.dir, k.layer + 1, out->det_counts);
             if (k.grad) {
                 wt += k.C;
                 for (int m = 0; m < nm; ++m) {
@@ -887,4 +999,17 @@ void orc_uniforms(uint64_t seed, uint64_t id, int64_t n, double* out) {
     orc_rng g;
     rng_init(&g, seed, id);
     for (int64_t i = 0; i < n; ++i) out[i] = rng_u(&g);
+}
+
+/* detector hit test and binning on given hit points (tests pin these against hand-worked
+   geometry): n points with layer (1-based); counts has the detector's bins */
+void orc_detector_hits(const smcrt_detector* det, int64_t n, const double* pos, const double* dir,
+                       const int* layer, uint64_t* counts) {
+    orc_det d;
+    size_t nb = det_setup(det, &d);
+    d.off = 0;
+    for (size_t b = 0; b < nb; ++b) counts[b] = 0;
+    for (int64_t i = 0; i < n; ++i)
+        det_record(&d, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                   mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), layer[i], counts);
 }

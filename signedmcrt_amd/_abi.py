@@ -12,7 +12,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsmcrt.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # status codes
 OK, EINVAL, ENOMEM, EDEVICE, ERANGE, EIO = 0, -1, -2, -3, -4, -5
@@ -60,12 +60,25 @@ class Source(C.Structure):
                 ("radius", C.c_double)]
 
 
+DET_CIRCLE, DET_ANNULUS, DET_CAMERA = 1, 2, 3
+MAX_DETECTORS = 16
+
+
+class Detector(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("layer", C.c_int32), ("nbins", C.c_int32),
+                ("track_history", C.c_int32), ("pos", C.c_double * 3), ("dir", C.c_double * 3),
+                ("p2", C.c_double * 3), ("p3", C.c_double * 3), ("radius1", C.c_double),
+                ("radius2", C.c_double), ("maxval", C.c_double)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("n_nodes", C.c_int32), ("n_regions", C.c_int32),
                 ("n_materials", C.c_int32),
                 ("nodes", C.POINTER(Node)), ("regions", C.POINTER(Region)),
                 ("materials", C.POINTER(Material)),
-                ("grid", Grid), ("source", Source)]
+                ("grid", Grid), ("source", Source),
+                ("n_detectors", C.c_int32), ("pad", C.c_int32),
+                ("detectors", C.POINTER(Detector))]
 
 
 class RunOpts(C.Structure):
@@ -96,7 +109,8 @@ class Stats(C.Structure):
 
 class Tallies(C.Structure):
     _fields_ = [("jmean", C.POINTER(C.c_double)), ("jmean_f32", C.POINTER(C.c_float)),
-                ("trace", C.POINTER(C.c_float)), ("stats", Stats)]
+                ("trace", C.POINTER(C.c_float)), ("detector_counts", C.POINTER(C.c_uint64)),
+                ("stats", Stats)]
 
 
 class GradOut(C.Structure):
@@ -110,15 +124,43 @@ class BatchSet(C.Structure):
                 ("first_packet", C.c_uint64), ("materials", C.POINTER(Material))]
 
 
+class Config(C.Structure):
+    """`smcrt_config`: the TOML input read by smcrt_config_load."""
+    _fields_ = [("source_name", C.c_char * 32), ("nphotons", C.c_int64),
+                ("has_position", C.c_int32), ("has_direction", C.c_int32),
+                ("position", C.c_double * 3), ("direction", C.c_double * 3),
+                ("point1", C.c_double * 3), ("point2", C.c_double * 3), ("point3", C.c_double * 3),
+                ("radius", C.c_double), ("beta", C.c_double), ("radius_hi", C.c_double),
+                ("annulus_type", C.c_char * 32),
+                ("nxg", C.c_int32), ("nyg", C.c_int32), ("nzg", C.c_int32), ("pad0", C.c_int32),
+                ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+                ("units", C.c_char * 16),
+                ("geom_name", C.c_char * 64),
+                ("tau", C.c_double), ("musb", C.c_double), ("muab", C.c_double),
+                ("musc", C.c_double), ("muac", C.c_double), ("hgg", C.c_double),
+                ("num_spheres", C.c_int32), ("pad1", C.c_int32),
+                ("fluence", C.c_char * 256), ("absorb", C.c_char * 256), ("render", C.c_char * 256),
+                ("render_geom", C.c_int32), ("overwrite", C.c_int32),
+                ("render_size", C.c_int32 * 3), ("pad2", C.c_int32),
+                ("iseed", C.c_int64), ("tev", C.c_int32), ("absorb_flag", C.c_int32),
+                ("load_checkpoint", C.c_int32), ("checkpoint_every_n", C.c_int32),
+                ("checkpoint_file", C.c_char * 256),
+                ("n_detectors", C.c_int32), ("pad3", C.c_int32),
+                ("detectors", Detector * MAX_DETECTORS),
+                ("history_file", C.c_char * 256)]
+
+
 STRUCTS = {"node": Node, "region": Region, "material": Material, "grid": Grid,
            "source": Source, "scene_desc": SceneDesc, "run_opts": RunOpts, "stats": Stats,
-           "tallies": Tallies, "grad_out": GradOut, "batch_set": BatchSet}
+           "tallies": Tallies, "grad_out": GradOut, "batch_set": BatchSet,
+           "detector": Detector, "config": Config}
 
 # every symbol include/smcrt.h declares (checked by tests/test_abi.py)
 EXPORTS = ["smcrt_abi_version", "smcrt_last_error", "smcrt_sizeof", "smcrt_scene_create", "smcrt_scene_destroy",
            "smcrt_scene_set_materials", "smcrt_run", "smcrt_run_device", "smcrt_evaluate",
            "smcrt_grad", "smcrt_batch_create", "smcrt_batch_run", "smcrt_batch_read",
-           "smcrt_batch_destroy", "smcrt_normalise_fluence", "smcrt_write_nrrd"]
+           "smcrt_batch_destroy", "smcrt_normalise_fluence", "smcrt_write_nrrd",
+           "smcrt_detector_bins", "smcrt_config_load", "smcrt_write_detector"]
 
 _lib = None
 
@@ -161,6 +203,9 @@ def lib() -> C.CDLL:
         "smcrt_write_nrrd": (C.c_int, [C.c_char_p, P(C.c_float), C.c_int32, C.c_int32, C.c_int32,
                                        C.c_char_p]),
         "smcrt_sizeof": (C.c_int64, [C.c_char_p]),
+        "smcrt_detector_bins": (C.c_int64, [C.c_void_p, C.c_int32]),
+        "smcrt_config_load": (C.c_int, [C.c_char_p, P(Config)]),
+        "smcrt_write_detector": (C.c_int, [C.c_char_p, P(Detector), P(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

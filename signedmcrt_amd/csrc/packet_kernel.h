@@ -65,8 +65,24 @@ struct KCold {
     const float* weights;  // nullptr: w = 1
     double* grad;          // [4][nmat] + [1] weighted tally
     const float* mat_mus;  // mus per adjoint material
+    // detectors: every ended history is offered to each (0: none, and no grid-exit continuation)
+    int n_det;
+    const struct DevDet* det;
+    unsigned long long* det_counts;  // every detector's bins, concatenated
 };
 typedef __attribute__((address_space(4))) const KCold CKCold;
+
+// A detector prepared for the hit test (src/detectors/detectors.f90).  Geometry in FP64 as the
+// reference keeps it; the test runs once per history.
+struct DevDet {
+    int kind, layer0, nb, pad;  // kind 1 circle, 2 annulus, 3 camera; 0-based layer; bins per axis
+    unsigned long long off;     // first bin in det_counts
+    double p[3], n[3];          // circle: position, unit direction; annulus: position; camera: p1, unit normal
+    double e1[3], e2[3];        // camera edges p2-p1, p3-p1
+    double a, b;                // circle: radius; annulus: r1, r2; camera: width, height
+    double bw, bwy;             // bin widths
+};
+typedef __attribute__((address_space(4))) const DevDet CDevDet;
 
 struct KParams {
     const DevNode* nodes;
@@ -100,6 +116,8 @@ struct Lane {
     float tau, taurun, dlay_old, dprev;
     int layer, st, bounces, nsc, set, tcount;
     bool iface;           // the current flight started at an interface
+    bool ghost;           // left the grid mid-flight: moves on, untallied, to the flight's end
+                          // or the next surface (only with detectors: the final position)
     unsigned iters;
     unsigned long long pid;
     Rng rng;
@@ -240,6 +258,66 @@ SMCRT_DEV float& adj_sc(const AdjLds& A, int which, int m, int nmat) {
 }
 SMCRT_DEV double& adj_g(const AdjLds& A, int which, int m, int nmat) {
     return A.g[(which * nmat + m) * A.stride + threadIdx.x];
+}
+
+// Offer an ended history to every detector: hit point = (final position, direction,
+// value = sqrt(x^2 + y^2), layer), src/kernelsMod.f90:302-306; tests and binning of
+// src/detectors/detectors.f90 and src/detectors/detector_base.f90:106-162.
+// 1-D bin (0-based): min(nint(value / width) + 1, bins), nint rounding half away from zero;
+// -1 (not counted) where the reference's index would leave the array (negative or NaN)
+SMCRT_DEV long long bin1d(double value, double width, int bins) {
+    const double q = value / width;
+    if (!(q >= 0.0)) return -1;
+    const long long i = q < 9.0e18 ? llround(q) + 1 : (long long)bins;
+    return (i < bins ? i : bins) - 1;
+}
+
+// camera axis bin (1-based): min(int(q) + 1, bins), wrapped to `bins` below 1; 0 for NaN
+SMCRT_DEV long long cam_bin(double q, int bins) {
+    if (q != q) return 0;
+    long long i = q >= 9.0e18 ? (long long)bins : (q <= -9.0e18 ? 0 : (long long)q + 1);
+    i = i < bins ? i : bins;
+    return i < 1 ? bins : i;
+}
+
+SMCRT_DEV void record_detectors(CKCold* C, const Lane& L) {
+    const double px = L.px, py = L.py, pz = L.pz, dx = L.dx, dy = L.dy, dz = L.dz;
+    const double value = sqrt(px * px + py * py);
+    for (int i = 0; i < C->n_det; ++i) {
+        CDevDet* D = (CDevDet*)C->det + i;
+        if (L.layer != D->layer0) continue;
+        long long bin = -1;
+        if (D->kind == 1) {  // circle: intersectCircle then t <= 5e-3
+            const double den = D->n[0] * dx + D->n[1] * dy + D->n[2] * dz;
+            if (den > 1e-6) {
+                const double t = ((D->p[0] - px) * D->n[0] + (D->p[1] - py) * D->n[1] + (D->p[2] - pz) * D->n[2]) / den;
+                if (t >= 0.0) {
+                    const double vx = px + dx * t - D->p[0], vy = py + dy * t - D->p[1], vz = pz + dz * t - D->p[2];
+                    if (sqrt(vx * vx + vy * vy + vz * vz) <= D->a && !(t > 5e-3))
+                        bin = bin1d(value, D->bw, D->nb);
+                }
+            }
+        } else if (D->kind == 2) {  // annulus
+            const double vx = px - D->p[0], vy = py - D->p[1], vz = pz - D->p[2];
+            const double r = sqrt(vx * vx + vy * vy + vz * vz);
+            if (r >= D->a && r <= D->b) bin = bin1d(value, D->bw, D->nb);
+        } else {  // camera
+            const double t = ((D->p[0] - px) * D->n[0] + (D->p[1] - py) * D->n[1] + (D->p[2] - pz) * D->n[2]) /
+                             (dx * D->n[0] + dy * D->n[1] + dz * D->n[2]);
+            if (t >= 0.0) {
+                const double vx = px + t * dx - D->p[0], vy = py + t * dy - D->p[1], vz = pz + t * dz - D->p[2];
+                const double q1 = (vx * D->e1[0] + vy * D->e1[1] + vz * D->e1[2]) / D->a;
+                const double q2 = (vx * D->e2[0] + vy * D->e2[1] + vz * D->e2[2]) / D->b;
+                if (q1 < D->a && q1 > 0.0 && q2 < D->b && q2 > 0.0) {
+                    // record_hit_2D_sub bins (final z + p1.x, final y + p1.y)
+                    const long long ix = cam_bin((pz + D->p[0]) / D->bw, D->nb);
+                    const long long iy = cam_bin((py + D->p[1]) / D->bwy, D->nb);
+                    if (ix > 0 && iy > 0) bin = (ix - 1) + (long long)D->nb * (iy - 1);
+                }
+            }
+        }
+        if (bin >= 0) atomicAdd(C->det_counts + D->off + bin, 1ull);
+    }
 }
 
 SMCRT_DEV int classify_escape(CKCold* C, const Lane& L) {
@@ -389,6 +467,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             L.nsc = 0;
             L.tcount = 0;
             L.iters = 0;
+            L.ghost = false;
             out[O_LAUNCHED]++;
             if (ADJ) {
                 adjC = 0.f;
@@ -447,6 +526,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             // src/inttau2.f90:85-97
             if (smin >= 0.f) end_kind = classify_escape(C, L);
             else if (dmin > eps) do_march = true;
+            else if (L.ghost) end_kind = classify_escape(C, L);  // outside the grid: ends at the surface
             else if (dmin <= L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
             else do_march = true;  // within eps but moving away from the surface: keep marching
         } else if (st0 == ST_CROSSED) {
@@ -579,8 +659,14 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             float d = dmin;
             if (W.flight_over) d = (L.tau - L.taurun) / kap;
             else L.taurun += t;
-            W.inside = walk_begin(P, L, W, d);
-            W.active = true;
+            if (L.ghost) {  // outside the grid: move without a walk (nothing to tally)
+                L.px += d * L.dx; L.py += d * L.dy; L.pz += d * L.dz;
+                if (W.flight_over) end_kind = classify_escape(C, L);
+                else L.st = ST_MARCHED;
+            } else {
+                W.inside = walk_begin(P, L, W, d);
+                W.active = true;
+            }
         }
 
         // ---- voxel walk: at most `cap` chords per lane this iteration ----
@@ -655,7 +741,15 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             }
             L.px += d * L.dx; L.py += d * L.dy; L.pz += d * L.dz;
             if (!W.inside) {
-                end_kind = classify_escape(C, L);
+                // left the grid (src/inttau2.f90:256-259,282-285): the outcome is fixed.  With
+                // detectors the packet moves on, untallied, to the end of its flight or the next
+                // surface, as the reference's loop does (:64-100), for its final position.
+                if (C->n_det > 0 && !W.flight_over) {
+                    L.ghost = true;
+                    L.st = ST_MARCHED;
+                } else {
+                    end_kind = classify_escape(C, L);
+                }
             } else if (W.flight_over) {
                 // albedo test (src/kernelsMod.f90:287-297)
                 const DevMat m = mt[L.layer];
@@ -705,6 +799,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             }
             pend_idx = -1;
             pend_li = -1;
+            if (C->n_det > 0 && end_kind != END_ERR) record_detectors(C, L);
             switch (end_kind) {
             case END_ABSORB: out[O_ABSORBED]++; trace_event(C, L, 5); break;
             case END_R: out[O_R]++; trace_event(C, L, 6); break;

@@ -30,6 +30,12 @@ static int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+// error reporting for the library's other translation units (smcrt_config.cpp)
+__attribute__((visibility("hidden"))) int smcrt_internal_fail(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+
 #define HIPCHECK(x)                                                                          \
     do {                                                                                     \
         hipError_t e_ = (x);                                                                 \
@@ -117,6 +123,10 @@ struct smcrt_scene {
     float circ_t[12];
     int circ_swap;
     float rdir[3];
+    // detectors (prepared for the device) and the offset of each one's bins
+    std::vector<smcrt_detector> dets;
+    std::vector<DevDet> ddets;
+    size_t det_bins = 0;
     // LDS tally tile (voxel box), chosen from a pilot run on first use
     bool tile_ready = false;
     int tile_t0[3] = {0, 0, 0}, tile_tn[3] = {0, 0, 0};
@@ -128,8 +138,58 @@ struct smcrt_scene {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
-        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold;
+        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold, d_det, d_det_counts;
 };
+
+static bool finite_all(const double* v, int n);
+
+// A detector as the reference constructs it (src/detectors/detectors.f90: init_circle_dect,
+// init_annulus_dect, init_camera): bin counts and widths, the circle's normalised direction
+// (src/parse/parse_detectors.f90: dir = dir%magnitude()), the camera's edges and unit normal.
+static int prepare_detector(const smcrt_detector& in, int idx, DevDet* out, size_t* bins) {
+    DevDet d;
+    memset(&d, 0, sizeof d);
+    if (in.kind < SMCRT_DET_CIRCLE || in.kind > SMCRT_DET_CAMERA)
+        return fail(SMCRT_EINVAL, "detector %d: Invalid detector type. Valid types are [circle, annulus, camera]", idx);
+    if (in.nbins < 0) return fail(SMCRT_EINVAL, "detector %d: nbins must be >= 0", idx);
+    if (!finite_all(in.pos, 3) || !finite_all(in.dir, 3) || !finite_all(in.p2, 3) || !finite_all(in.p3, 3) ||
+        !std::isfinite(in.radius1) || !std::isfinite(in.radius2) || !std::isfinite(in.maxval))
+        return fail(SMCRT_EINVAL, "detector %d: non-finite parameter", idx);
+    d.kind = in.kind;
+    d.layer0 = in.layer - 1;
+    d.nb = in.nbins + 1;
+    for (int j = 0; j < 3; ++j) d.p[j] = in.pos[j];
+    if (in.kind == SMCRT_DET_CIRCLE) {
+        const double l = std::sqrt(in.dir[0] * in.dir[0] + in.dir[1] * in.dir[1] + in.dir[2] * in.dir[2]);
+        for (int j = 0; j < 3; ++j) d.n[j] = in.dir[j] / l;
+        d.a = in.radius1;
+        d.bw = in.nbins == 0 ? 1.0 : in.maxval / (double)(in.nbins - 1);
+        *bins = (size_t)d.nb;
+    } else if (in.kind == SMCRT_DET_ANNULUS) {
+        if (!(in.radius2 > in.radius1))
+            return fail(SMCRT_EINVAL, "detector %d: Radii are invalid (Expected radius2 (%g) > radius 1 (%g))", idx,
+                        in.radius2, in.radius1);
+        d.a = in.radius1;
+        d.b = in.radius2;
+        d.bw = in.nbins == 0 ? 1.0 : in.maxval / (double)in.nbins;
+        *bins = (size_t)d.nb;
+    } else {
+        double e1[3], e2[3];
+        for (int j = 0; j < 3; ++j) { e1[j] = in.p2[j] - in.pos[j]; e2[j] = in.p3[j] - in.pos[j]; }
+        // n = e2 x e1, normalised
+        double n[3] = {e2[1] * e1[2] - e2[2] * e1[1], e2[2] * e1[0] - e2[0] * e1[2], e2[0] * e1[1] - e2[1] * e1[0]};
+        const double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (!(nl > 0)) return fail(SMCRT_EINVAL, "detector %d: camera corners are collinear", idx);
+        for (int j = 0; j < 3; ++j) { d.e1[j] = e1[j]; d.e2[j] = e2[j]; d.n[j] = n[j] / nl; }
+        d.a = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        d.b = std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        d.bw = in.nbins == 0 ? 1.0 : in.maxval / (double)d.nb;
+        d.bwy = d.bw;
+        *bins = (size_t)d.nb * (size_t)d.nb;
+    }
+    *out = d;
+    return SMCRT_OK;
+}
 
 static int n_values_pushed(int op) { return op == SMCRT_OP_PRIM ? 1 : (op == SMCRT_OP_CSG ? -1 : 0); }
 
@@ -183,7 +243,8 @@ extern "C" int64_t smcrt_sizeof(const char* name) {
         {"source", sizeof(smcrt_source)}, {"scene_desc", sizeof(smcrt_scene_desc)},
         {"run_opts", sizeof(smcrt_run_opts)}, {"stats", sizeof(smcrt_stats)},
         {"tallies", sizeof(smcrt_tallies)}, {"grad_out", sizeof(smcrt_grad_out)},
-        {"batch_set", sizeof(smcrt_batch_set)}};
+        {"batch_set", sizeof(smcrt_batch_set)}, {"detector", sizeof(smcrt_detector)},
+        {"config", sizeof(smcrt_config)}};
     for (auto& e : t)
         if (!strcmp(e.n, name)) return e.s;
     return -1;
@@ -326,6 +387,29 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
     if (src.kind == SMCRT_SRC_POINT) { s->rdir[0] = 0; s->rdir[1] = 0; s->rdir[2] = -1; }
     else for (int j = 0; j < 3; ++j) s->rdir[j] = (float)(src.dir[j] / dn);
 
+    // detectors
+    if (d->n_detectors < 0 || d->n_detectors > SMCRT_MAX_DETECTORS) {
+        delete s;
+        return fail(SMCRT_ERANGE, "n_detectors %d outside [0, %d]", d->n_detectors, SMCRT_MAX_DETECTORS);
+    }
+    if (d->n_detectors > 0 && !d->detectors) {
+        delete s;
+        return fail(SMCRT_EINVAL, "null detectors");
+    }
+    for (int i = 0; i < d->n_detectors; ++i) {
+        DevDet dd;
+        size_t nb = 0;
+        int rc2 = prepare_detector(d->detectors[i], i, &dd, &nb);
+        if (rc2) {
+            delete s;
+            return rc2;
+        }
+        dd.off = s->det_bins;
+        s->det_bins += nb;
+        s->dets.push_back(d->detectors[i]);
+        s->ddets.push_back(dd);
+    }
+
     // epsilons: the march/crossing epsilon is scaled to FP32 resolution at the scene's size
     // (2^-17 of the largest grid half-size, about 64 ulp); the kill threshold after emission or
     // a scatter keeps the reference's 1e-8 (src/inttau2.f90:44)
@@ -342,7 +426,8 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
         (void)hipSetDevice(s->device);
         DevBuf* bufs[] = {&s->d_nodes, &s->d_regions, &s->d_mats, &s->d_region_mat, &s->d_mat_mus, &s->d_work,
                           &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights,
-                          &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum, &s->d_cold};
+                          &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum, &s->d_cold, &s->d_det,
+                          &s->d_det_counts};
         for (DevBuf* b : bufs) b->release();
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -350,6 +435,12 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
     }
     delete s;
     return SMCRT_OK;
+}
+
+extern "C" int64_t smcrt_detector_bins(const smcrt_scene* s, int32_t i) {
+    if (!s || i < 0 || i >= (int32_t)s->ddets.size()) return 0;
+    const size_t end = (size_t)i + 1 < s->ddets.size() ? s->ddets[i + 1].off : s->det_bins;
+    return (int64_t)(end - s->ddets[i].off);
 }
 
 extern "C" int smcrt_scene_set_materials(smcrt_scene* s, const smcrt_material* m, int32_t n) {
@@ -388,6 +479,11 @@ static int ensure_device(smcrt_scene* s) {
         HIPCHECK(hipMemcpy(s->d_nodes.p, s->nodes.data(), s->nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(s->d_regions.p, s->regions.data(), s->regions.size() * sizeof(DevRegion), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(s->d_region_mat.p, s->region_mat.data(), s->region_mat.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!s->ddets.empty()) {
+            if ((rc = s->d_det.ensure(s->ddets.size() * sizeof(DevDet)))) return rc;
+            if ((rc = s->d_det_counts.ensure(s->det_bins * sizeof(unsigned long long)))) return rc;
+            HIPCHECK(hipMemcpy(s->d_det.p, s->ddets.data(), s->ddets.size() * sizeof(DevDet), hipMemcpyHostToDevice));
+        }
         s->uploaded = true;
     }
     if (s->mats_dirty) {
@@ -587,6 +683,14 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     P.refill_min = sh.refill_min;
     Cc.weights = d_w;
     Cc.grad = s->d_grad.as<double>();
+    // detectors (not in adjoint launches: their final-position continuation changes nothing the
+    // adjoint scores)
+    if (!adj && !s->ddets.empty()) {
+        Cc.n_det = (int)s->ddets.size();
+        Cc.det = s->d_det.as<DevDet>();
+        Cc.det_counts = s->d_det_counts.as<unsigned long long>();
+        k_fill_u64<<<(unsigned)fill_blocks(s->det_bins), 256, 0, stream>>>(Cc.det_counts, s->det_bins);
+    }
     {
         int rc = s->d_cold.ensure(sizeof(KCold));
         if (rc) return rc;
@@ -693,6 +797,10 @@ extern "C" int smcrt_run(smcrt_scene* s, const smcrt_run_opts* o, smcrt_tallies*
         out->stats.tally_sum = sum;
         if (out->jmean_f32)
             for (size_t i = 0; i < nv; ++i) out->jmean_f32[i] = (float)host[i];
+    }
+    if (out->detector_counts && s->det_bins) {
+        HIPCHECK(hipMemcpy(out->detector_counts, s->d_det_counts.p, s->det_bins * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost));
     }
     if (out->trace && o->trace_packets > 0 && o->trace_events > 0) {
         HIPCHECK(hipMemcpy(out->trace, s->d_trace.p, (size_t)o->trace_packets * o->trace_events * 4 * sizeof(float),

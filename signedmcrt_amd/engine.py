@@ -22,6 +22,7 @@ class RunResult:
     jmean: Optional[np.ndarray]  # raw FP64 path-length sums, shape (nx, ny, nz), Fortran order
     stats: dict
     trace: Optional[np.ndarray] = None
+    detectors: Optional[list] = None  # per detector: its bins (see signedmcrt_amd.detectors)
 
     @property
     def n(self) -> int:
@@ -122,9 +123,22 @@ class Engine:
         if trace_packets:
             tr = np.zeros(trace_packets * trace_events * 4, dtype=np.float32)
             t.trace = _fptr(tr)
+        dets = self.scene.detectors
+        counts = None
+        if dets:
+            nbins = [int(self._lib.smcrt_detector_bins(self.handle, i)) for i in range(len(dets))]
+            counts = np.zeros(sum(nbins), dtype=np.uint64)
+            t.detector_counts = counts.ctypes.data_as(C.POINTER(C.c_uint64))
         A.check(self._lib.smcrt_run(self.handle, C.byref(o), C.byref(t)))
+        det_out = None
+        if dets:
+            det_out, off = [], 0
+            for d, nb in zip(dets, nbins):
+                det_out.append(counts[off:off + nb].reshape(d.shape, order="F").copy())
+                off += nb
         return RunResult(jm.reshape(g.shape, order="F") if tally else None, t.stats.as_dict(),
-                         tr.reshape(trace_packets, trace_events, 4) if trace_packets else None)
+                         tr.reshape(trace_packets, trace_events, 4) if trace_packets else None,
+                         det_out)
 
     def run_device(self, n: int, d_jmean_ptr: int, seed: int = 123456789, first_packet: int = 0,
                    flags: int = 0, stream: int = 0, **shape) -> dict:

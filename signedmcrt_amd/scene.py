@@ -80,12 +80,15 @@ class source:
 class Scene:
     """A flattened scene plus the ctypes buffers backing its descriptor."""
 
-    def __init__(self, array: Sequence[SDF], grid: cart_grid, src: source):
+    def __init__(self, array: Sequence[SDF], grid: cart_grid, src: source, detectors: Sequence = ()):
         if not array:
             raise ValueError("scene needs at least one SDF")
         self.array = list(array)
         self.grid = grid
         self.source = src
+        self.detectors = list(detectors)
+        if len(self.detectors) > A.MAX_DETECTORS:
+            raise ValueError(f"at most {A.MAX_DETECTORS} detectors")
         self.materials: List[mono] = []
         nodes: List[A.Node] = []
         regions: List[A.Region] = []
@@ -111,6 +114,9 @@ class Scene:
         g.xmax, g.ymax, g.zmax = grid.xmax, grid.ymax, grid.zmax
         d.grid = g
         d.source = src.to_c()
+        self._dets = (A.Detector * max(1, len(self.detectors)))(*[x.to_c() for x in self.detectors])
+        d.n_detectors = len(self.detectors)
+        d.detectors = C.cast(self._dets, C.POINTER(A.Detector))
         self.desc = d
 
     def _material_index(self, m: mono) -> int:

@@ -142,6 +142,16 @@ REGISTRY: Dict[str, callable] = {
 }
 
 
+# [geometry] table keys each registered builder takes from a TOML input: the reference's setup
+# routines read `tau` (setup_scat_test, setup_scat_test2), `hgg` (setup_scat_test2) and
+# `num_spheres` (setup_sphere_scene) from it (src/setupGeometry.f90); the others take none.
+TOML_PARAMS: Dict[str, Tuple[str, ...]] = {
+    "scat_test": ("tau",),
+    "scat_test2": ("tau", "hgg"),
+    "sphere_scene": ("num_spheres",),
+}
+
+
 def build(geom_name: str, **params) -> Tuple[Scene, int]:
     """Scene and default packet count for a registered geometry ("no such routine" otherwise,
     src/setup.f90:50-51)."""

@@ -73,7 +73,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=int, default=0, help="packets per step (0: the workload's)")
-    ap.add_argument("--workload", default="skin_vessels", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="skin_vessels", choices=sorted(WORKLOADS) + ["batched"])
+    ap.add_argument("--sets", type=int, default=256, help="batched: property sets")
     ap.add_argument("--no-tune", action="store_true", help="library default launch shape")
     ap.add_argument("--shape", default="", help="launch shape as JSON (skips the autotuner), e.g. "
                     "'{\"block\": 256, \"chunk\": 256, \"walk_cap\": 8, \"refill_min\": 1, \"tile\": true}'")
@@ -107,6 +108,9 @@ def main():
 
     from signedmcrt_amd import roofline, scenes
     from signedmcrt_amd.engine import Engine
+
+    if args.workload == "batched":
+        return bench_batched(args)
 
     n_default, description, fd_mats = WORKLOADS[args.workload]
     scene, _ = scenes.build(args.workload)
@@ -241,6 +245,82 @@ def main():
         "cpu_baseline": cpu,
         "grad": grad,
         "autotune": tuned,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def batched_sets(n_sets: int, seed: int = 20240917):
+    """The batched workload's property sets, drawn from a seeded generator: mu_a log-uniform
+    0.01-1/cm, mu_s uniform 10-300/cm, g uniform 0.7-0.95, n uniform 1.33-1.45, packet budget
+    log-uniform 1e5-1e7; even sets on the slab geometry, odd sets on the refracting sphere
+    (the drawn properties go to the scattering medium: the slab, the sphere's cube)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_sets):
+        mua = float(np.exp(rng.uniform(np.log(0.01), np.log(1.0))))
+        mus = float(rng.uniform(10.0, 300.0))
+        g = float(rng.uniform(0.7, 0.95))
+        n = float(rng.uniform(1.33, 1.45))
+        budget = int(np.exp(rng.uniform(np.log(1e5), np.log(1e7))))
+        out.append((i % 2, mua, mus, g, n, budget))
+    return out
+
+
+def bench_batched(args):
+    """One HIP-graph replay = every set's packets in one persistent launch per geometry."""
+    import numpy as np
+    import torch
+    from signedmcrt_amd import scenes
+    from signedmcrt_amd.batch import Batch, BatchSet
+    from signedmcrt_amd.engine import Engine
+
+    engines = [Engine(scenes.build("slab", nxyz=64)[0]), Engine(scenes.build("refracting_sphere", nxyz=64)[0])]
+    medium = [0, 1]  # index of the scattering medium among each scene's materials
+    sets = []
+    first = 0
+    for geo, mua, mus, g, n, budget in batched_sets(args.sets):
+        mats = engines[geo].scene.materials
+        MUS = [m.mus for m in mats]
+        MUA = [m.mua for m in mats]
+        HG = [m.hgg for m in mats]
+        NN = [m.n for m in mats]
+        k = medium[geo]
+        MUS[k], MUA[k], HG[k], NN[k] = mus, mua, g, n
+        sets.append(BatchSet(geo, MUS, MUA, budget, first_packet=first, hgg=HG, n=NN))
+        first += budget
+    total = sum(s.n_packets for s in sets)
+    shape = {}
+    tuned = None
+    if not args.no_tune:
+        res = engines[1].tune(use_cache=False)
+        shape = {k: v for k, v in res.shape.items() if k != "tile"}  # batches tally straight to HBM
+        tuned = {"shape": res.shape, "tuned_on": "refracting_sphere 64^3", "speedup_vs_default": res.speedup}
+    b = Batch(engines, sets, seed=123456789, **shape)
+    for _ in range(args.warmup):
+        b.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = total * args.steps / wall
+    st = [b.read(i)[1] for i in range(len(sets))]
+    launched = sum(x["launched"] for x in st)
+    assert launched == total, (launched, total)
+    frac = {k: sum(x[k] for x in st) / total for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface")}
+    budgets = np.array([s.n_packets for s in sets])
+    out = {
+        "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched", "sets": len(sets), "packets_per_step": int(total),
+                   "budget_min": int(budgets.min()), "budget_max": int(budgets.max()),
+                   "grids": "64^3 FP64 per set", "geometries": ["slab", "refracting_sphere"],
+                   "launch": "one HIP graph: fill kernels + one persistent packet-kernel node per geometry",
+                   "launch_shape": shape or "library default"},
+        "fractions": frac, "autotune": tuned, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

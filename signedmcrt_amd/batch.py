@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -27,6 +27,8 @@ class BatchSet:
     mua: Sequence[float]
     n_packets: int
     first_packet: int = 0
+    hgg: Optional[Sequence[float]] = None   # per material (None: the scene's)
+    n: Optional[Sequence[float]] = None
 
 
 class Batch:
@@ -38,7 +40,10 @@ class Batch:
         self._mats = []
         arr = (A.BatchSet * len(sets))()
         for i, s in enumerate(sets):
-            m = engines[s.scene].scene.material_array(np.asarray(s.mus, float), np.asarray(s.mua, float))
+            m = engines[s.scene].scene.material_array(
+                np.asarray(s.mus, float), np.asarray(s.mua, float),
+                None if s.hgg is None else np.asarray(s.hgg, float),
+                None if s.n is None else np.asarray(s.n, float))
             self._mats.append(m)
             arr[i].scene = s.scene
             arr[i].n_packets = int(s.n_packets)

@@ -131,13 +131,15 @@ class Scene:
             self._mats[i].mus, self._mats[i].mua = m.mus, m.mua
             self._mats[i].hgg, self._mats[i].n = m.hgg, m.n
 
-    def material_array(self, mus: Optional[np.ndarray] = None, mua: Optional[np.ndarray] = None):
-        """A ctypes material table with mus/mua optionally replaced (used by FD and the adjoint)."""
+    def material_array(self, mus: Optional[np.ndarray] = None, mua: Optional[np.ndarray] = None,
+                       hgg: Optional[np.ndarray] = None, n: Optional[np.ndarray] = None):
+        """A ctypes material table with properties optionally replaced (FD, adjoint, batches)."""
         arr = (A.Material * len(self.materials))()
         for i, m in enumerate(self.materials):
             arr[i].mus = float(mus[i]) if mus is not None else m.mus
             arr[i].mua = float(mua[i]) if mua is not None else m.mua
-            arr[i].hgg, arr[i].n = m.hgg, m.n
+            arr[i].hgg = float(hgg[i]) if hgg is not None else m.hgg
+            arr[i].n = float(n[i]) if n is not None else m.n
         return arr
 
     @property

@@ -32,8 +32,8 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (published)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E (published)
 METRIC = "photon packets/s @ fixed fluence error (1 GPU vs host CPU); grad cost vs FD"
-# workload -> (packets per step, description, FD parameters: materials whose (mu_a, mu_s) the
-# finite-difference comparison perturbs)
+# workload -> (packets per step, description, gradient: (materials the adjoint differentiates,
+# number of parameters central finite differences would perturb) or None)
 WORKLOADS = {
     "slab": (1_000_000, "slab 1x1x0.2 cm box mus=100 mua=0.1 g=0.9 n=1.37 in air, pencil beam, "
              "100^3 grid", None),
@@ -42,7 +42,11 @@ WORKLOADS = {
     "skin_vessels": (100_000_000, "5 stacked tissue boxes (0.01-0.5 cm, mua 0.02-5, mus 20-300, "
                      "g 0.8-0.95, n 1.37-1.45, seeded) + 8 blood cylinders r=0.01-0.05 mua=20 "
                      "merged by SmoothUnion k=0.01, air, pencil beam, 256^3 grid",
-                     (1, 2, 3, 4, 5)),
+                     (None, 10)),  # every material; FD on the 5 layers' (mu_a, mu_s)
+    "polygon_regions": (100_000_000, "16 seeded closed polygons x 125 edges (2000 segments) "
+                        "extruded to 0.5 cm in a 4x4x0.5 cm tissue slab (16 materials + background), "
+                        "uniform-grid polygon accelerator, uniform top illumination, 400x400x50 grid",
+                        (list(range(16)), 16)),  # the 16 polygons' mu_a
 }
 
 
@@ -75,6 +79,9 @@ def main():
     ap.add_argument("--packets", type=int, default=0, help="packets per step (0: the workload's)")
     ap.add_argument("--workload", default="skin_vessels", choices=sorted(WORKLOADS) + ["batched"])
     ap.add_argument("--sets", type=int, default=256, help="batched: property sets")
+    ap.add_argument("--fit-iters", type=int, default=0,
+                    help="polygon_regions: inverse-fit iterations to report (0: skip)")
+    ap.add_argument("--fit-packets", type=int, default=20_000_000)
     ap.add_argument("--no-tune", action="store_true", help="library default launch shape")
     ap.add_argument("--shape", default="", help="launch shape as JSON (skips the autotuner), e.g. "
                     "'{\"block\": 256, \"chunk\": 256, \"walk_cap\": 8, \"refill_min\": 1, \"tile\": true}'")
@@ -112,7 +119,7 @@ def main():
     if args.workload == "batched":
         return bench_batched(args)
 
-    n_default, description, fd_mats = WORKLOADS[args.workload]
+    n_default, description, grad_spec = WORKLOADS[args.workload]
     scene, _ = scenes.build(args.workload)
     g = scene.grid
     nv = g.nxg * g.nyg * g.nzg
@@ -173,22 +180,43 @@ def main():
     # a launch lasts at least as long as its longest history, ~1e5 scatters in a low-absorption
     # layer, which for small launches can dominate the kernel time.)
     grad = None
-    if fd_mats and args.grad_steps > 0:
+    if grad_spec and args.grad_steps > 0:
+        gmats, p = grad_spec
         ng = n
         fwd_ms = stats[0]["kernel_ms"]
-        eng.grad(min(ng, 1_000_000), seed=77)  # warm (code load, LDS sizing)
-        gr = [eng.grad(ng, seed=123456789, first_packet=0) for _ in range(args.grad_steps)]
+        eng.grad(min(ng, 1_000_000), seed=77, materials=gmats)  # warm (code load, LDS sizing)
+        gr = [eng.grad(ng, seed=123456789, first_packet=0, materials=gmats) for _ in range(args.grad_steps)]
         grad_ms = float(np.mean([x.stats["kernel_ms"] for x in gr]))
         for k in ("launched", "absorbed", "escaped_r", "scatters"):
             assert gr[0].stats[k] == stats[0][k], ("adjoint and forward histories differ", k)
-        p = 2 * len(fd_mats)
+        nsel = scene.n_materials if gmats is None else len(gmats)
         grad = {"packets": ng, "forward_kernel_ms": fwd_ms, "grad_kernel_ms": grad_ms,
                 "grad_cost_vs_forward": grad_ms / fwd_ms,
                 "fd_parameters": p, "fd_forward_launches": 2 * p,
                 "speedup_vs_central_fd": 2 * p * fwd_ms / grad_ms,
-                "derivatives_per_launch": 4 * scene.n_materials,
+                "derivatives_per_launch": 4 * nsel,
                 "d_rd_dmua": [float(v) for v in gr[0].d_r_dmua],
                 "d_rd_dmus": [float(v) for v in gr[0].d_r_dmus]}
+
+    fit = None
+    if args.workload == "polygon_regions" and args.fit_iters > 0:
+        from signedmcrt_amd.inverse import fit_mua
+        truth = np.array([scene.materials[k].mua for k in range(16)])
+        nf = args.fit_packets
+        meas = np.zeros(nv)
+        eng.run_device(nf, d_grid.zero_().data_ptr(), seed=4242)
+        meas = d_grid.cpu().numpy().reshape(g.shape, order="F") / nf
+        fscene, _ = scenes.build(args.workload, mua=[0.5] * 16)
+        feng = Engine(fscene, shape=eng.shape)
+        t_fit = time.perf_counter()
+        hist = fit_mua(feng, meas, materials=list(range(16)), mua0=0.5, n_packets=nf,
+                       iters=args.fit_iters, lr=0.1, block=10)
+        err = np.abs(hist.final / truth - 1)
+        fit = {"iters": args.fit_iters, "packets_per_launch": nf, "seconds": time.perf_counter() - t_fit,
+               "loss_first": hist.loss[0], "loss_last": hist.loss[-1],
+               "rel_err_start": float(np.median(np.abs(0.5 / truth - 1))),
+               "rel_err_median": float(np.median(err)), "rel_err_max": float(err.max()),
+               "truth": truth.tolist(), "fit": hist.final.tolist()}
 
     cpu = None
     if not args.no_cpu and B >= 2:
@@ -245,6 +273,7 @@ def main():
         "cpu_baseline": cpu,
         "grad": grad,
         "autotune": tuned,
+        "inverse_fit": fit,
     }
     print(json.dumps(out), flush=True)
 

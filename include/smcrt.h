@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SMCRT_ABI_VERSION 2
+#define SMCRT_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------------------------- */
 enum {
@@ -77,7 +77,13 @@ enum {
     SMCRT_CAPSULE = 7,  /* a.x a.y a.z b.x b.y b.z r                                       */
     SMCRT_CONE = 8,     /* a.x a.y a.z b.x b.y b.z ra rb                                   */
     SMCRT_EGG = 9,      /* r1, r2, h                                                      */
-    SMCRT_PLANE = 10    /* a.x a.y a.z (unit normal)                                       */
+    SMCRT_PLANE = 10,   /* a.x a.y a.z (unit normal)                                       */
+    SMCRT_POLYGON = 11  /* first vertex, vertex count: a closed polygon in the xy plane from
+                           smcrt_scene_desc.poly_vertices (no reference counterpart: the
+                           reference's `segment` is a tube, not a region).  Signed 2-D distance
+                           (z ignored; wrap it in SMCRT_EXTRUDE for a prism), evaluated through a
+                           per-polygon uniform-grid accelerator built by smcrt_scene_create.
+                           The polygon must be simple; either orientation.                  */
 };
 /* CSG operators (src/sdfs/sdfModifiers.f90:412-475) */
 enum { SMCRT_UNION = 1, SMCRT_SMOOTH_UNION = 2, SMCRT_SUBTRACTION = 3, SMCRT_INTERSECTION = 4 };
@@ -169,8 +175,9 @@ typedef struct smcrt_scene_desc {
     smcrt_grid grid;
     smcrt_source source;
     int32_t n_detectors;  /* 0..SMCRT_MAX_DETECTORS                                       */
-    int32_t pad;
+    int32_t n_poly_vertices; /* polygon vertices (x, y pairs) referenced by SMCRT_POLYGON nodes */
     const smcrt_detector* detectors;
+    const double* poly_vertices; /* 2 * n_poly_vertices doubles (may be NULL if none)        */
 } smcrt_scene_desc;
 
 /* ---- run options, statistics, tallies -------------------------------------------------- */
@@ -198,7 +205,9 @@ typedef struct smcrt_run_opts {
     int32_t walk_cap;      /* voxel chords a lane walks per kernel-loop iteration, 0 -> default */
     int32_t refill_min;    /* regeneration policy: a wavefront refills its idle lanes once at
                               least this many are idle (1 = at once; 0 -> default)            */
-    int32_t reserved[3];
+    uint32_t grad_mask_lo; /* smcrt_grad: materials whose derivatives are computed, bit m =   */
+    uint32_t grad_mask_hi; /* material m (0 and 0: every material); at most 16 set            */
+    int32_t reserved;
 } smcrt_run_opts;
 
 /* How a packet ended.  A+R+T+K = launched. */

@@ -229,6 +229,27 @@ static double mod_post(int kind, const double* P, v3 p, double d) {
     return d;
 }
 
+/* closed polygon in the xy plane (SMCRT_POLYGON), brute force over every edge: distance to the
+   nearest edge, sign by the crossing-number test of Inigo Quilez's published polygon distance
+   (negative inside, either orientation).  The library evaluates the same shape through its
+   uniform-grid accelerator with a local sign rule; this is the independent check. */
+static double polygon_brute(const smcrt_scene_desc* s, const double* P, v3 p) {
+    const int f = (int)P[0], c = (int)P[1];
+    const double* v = s->poly_vertices + 2 * (size_t)f;
+    double d = INFINITY, sg = 1.0;
+    for (int i = 0, j = c - 1; i < c; j = i, i++) {
+        const double ex = v[2 * j] - v[2 * i], ey = v[2 * j + 1] - v[2 * i + 1];
+        const double wx = p.x - v[2 * i], wy = p.y - v[2 * i + 1];
+        double t = (wx * ex + wy * ey) / (ex * ex + ey * ey);
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double bx = wx - ex * t, by = wy - ey * t;
+        d = fmin(d, bx * bx + by * by);
+        const int c1 = p.y >= v[2 * i + 1], c2 = p.y < v[2 * j + 1], c3 = ex * wy > ey * wx;
+        if ((c1 && c2 && c3) || (!c1 && !c2 && !c3)) sg = -sg;
+    }
+    return sg * sqrt(d);
+}
+
 /* evaluate region r at p: the postfix program of smcrt.h */
 static double region_eval(const smcrt_scene_desc* s, int r, v3 p) {
     double vs[SMCRT_MAX_VALUE_STACK + 1];
@@ -239,7 +260,11 @@ static double region_eval(const smcrt_scene_desc* s, int r, v3 p) {
     for (int i = 0; i < R->n_nodes; ++i) {
         const smcrt_node* n = &s->nodes[R->first_node + i];
         switch (n->op) {
-        case SMCRT_OP_PRIM: vs[nv++] = prim_eval(n->kind, n->p, xform(n->t, ps[np - 1])); break;
+        case SMCRT_OP_PRIM: {
+            v3 q = xform(n->t, ps[np - 1]);
+            vs[nv++] = n->kind == SMCRT_POLYGON ? polygon_brute(s, n->p, q) : prim_eval(n->kind, n->p, q);
+            break;
+        }
         case SMCRT_OP_CSG: { double d2 = vs[--nv]; double d1 = vs[--nv]; vs[nv++] = csg(n->kind, d1, d2, n->k); break; }
         case SMCRT_OP_MOD_BEGIN: { v3 q = mod_pos(n->kind, n->p, ps[np - 1]); ps[np++] = q; break; }
         case SMCRT_OP_MOD_END: { --np; vs[nv - 1] = mod_post(n->kind, n->p, ps[np - 1], vs[nv - 1]); break; }

@@ -12,14 +12,14 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsmcrt.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # status codes
 OK, EINVAL, ENOMEM, EDEVICE, ERANGE, EIO = 0, -1, -2, -3, -4, -5
 
 # node ops / kinds (smcrt.h)
 OP_PRIM, OP_CSG, OP_MOD_BEGIN, OP_MOD_END = 1, 2, 3, 4
-SPHERE, BOX, TORUS, CYLINDER, TRIPRISM, SEGMENT, CAPSULE, CONE, EGG, PLANE = range(1, 11)
+SPHERE, BOX, TORUS, CYLINDER, TRIPRISM, SEGMENT, CAPSULE, CONE, EGG, PLANE, POLYGON = range(1, 12)
 UNION, SMOOTH_UNION, SUBTRACTION, INTERSECTION = 1, 2, 3, 4
 EXTRUDE, REVOLUTION, ONION, ELONGATE, TWIST, BEND = range(1, 7)
 SRC_PENCIL, SRC_UNIFORM, SRC_CIRCULAR, SRC_POINT = 1, 2, 3, 4
@@ -77,8 +77,8 @@ class SceneDesc(C.Structure):
                 ("nodes", C.POINTER(Node)), ("regions", C.POINTER(Region)),
                 ("materials", C.POINTER(Material)),
                 ("grid", Grid), ("source", Source),
-                ("n_detectors", C.c_int32), ("pad", C.c_int32),
-                ("detectors", C.POINTER(Detector))]
+                ("n_detectors", C.c_int32), ("n_poly_vertices", C.c_int32),
+                ("detectors", C.POINTER(Detector)), ("poly_vertices", C.POINTER(C.c_double))]
 
 
 class RunOpts(C.Structure):
@@ -86,7 +86,8 @@ class RunOpts(C.Structure):
                 ("flags", C.c_uint32), ("max_bounces", C.c_int32), ("block_size", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
                 ("trace_packets", C.c_int32), ("trace_events", C.c_int32),
-                ("walk_cap", C.c_int32), ("refill_min", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("walk_cap", C.c_int32), ("refill_min", C.c_int32),
+                ("grad_mask_lo", C.c_uint32), ("grad_mask_hi", C.c_uint32), ("reserved", C.c_int32)]
 
 
 STAT_COUNTERS = ["launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",

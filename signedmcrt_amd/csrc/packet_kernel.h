@@ -64,7 +64,7 @@ struct KCold {
     // adjoint
     const float* weights;  // nullptr: w = 1
     double* grad;          // [4][nmat] + [1] weighted tally
-    const float* mat_mus;  // mus per adjoint material
+    const float* mat_mus;  // mus per adjoint slot
     // detectors: every ended history is offered to each (0: none, and no grid-exit continuation)
     int n_det;
     const struct DevDet* det;
@@ -87,6 +87,7 @@ typedef __attribute__((address_space(4))) const DevDet CDevDet;
 struct KParams {
     const DevNode* nodes;
     const DevRegion* regions;
+    const float* accel;  // polygon accelerators (nullptr: none)
     const DevMat* mats;  // [nsets][nreg] region-indexed material constants
     int nreg, nmat;      // regions, adjoint materials
     // grid
@@ -233,14 +234,15 @@ SMCRT_DEV float fresnel(float ix, float iy, float iz, float nx, float ny, float 
 // Evaluate, per lane, the region r_lane at p (only lanes with `active` set).  Waterfall over the
 // distinct region ids in the wavefront so each evaluation runs the wave-uniform interpreter.
 template <class T>
-SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane, const V3<T>& p) {
+SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane, const V3<T>& p,
+                             const float* __restrict__ accel) {
     T res = g_const<T>(0.f);
     uint64_t pending = wave_ballot(active);
     while (pending) {
         int leader = __builtin_ctzll(pending);
         int r = __builtin_amdgcn_readlane(r_lane, leader);
         bool mine = active && r_lane == r;
-        if (mine) res = region_eval<T>(nodes, regions[r].first, regions[r].count, p);
+        if (mine) res = region_eval<T>(nodes, regions[r].first, regions[r].count, p, accel);
         pending &= ~wave_ballot(mine);
     }
     return res;
@@ -486,7 +488,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         if (sweeping) {
             V3<float> p{L.px, L.py, L.pz};
             for (int r = 0; r < P.nreg; ++r) {
-                float d = region_eval<float>(nodes, regions[r].first, regions[r].count, p);
+                float d = region_eval<float>(nodes, regions[r].first, regions[r].count, p, P.accel);
                 dmin = fminf(dmin, fabsf(d));
                 smin = fminf(smin, d);
                 if (r == 0) d0 = d;
@@ -587,7 +589,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 const bool p0 = pass == 0;
                 V3<D1> pd{D1{p0 ? L.opx : L.px, pass == 1 ? 1.f : 0.f}, D1{p0 ? L.opy : L.py, pass == 2 ? 1.f : 0.f},
                           D1{p0 ? L.opz : L.pz, pass == 3 ? 1.f : 0.f}};
-                const D1 r = eval_lane_region<D1>(nodes, regions, active, p0 ? fn : use_region, pd);
+                const D1 r = eval_lane_region<D1>(nodes, regions, active, p0 ? fn : use_region, pd, P.accel);
                 if (p0) {
                     if (want_dstmp) {
                         const float dt = r.v, dl = dlay, dn = dfirst;
@@ -729,7 +731,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         if (walk_done) {
             W.active = false;
             const float d = W.d;
-            if (ADJ) {
+            if (ADJ && mt[L.layer].mat >= 0) {  // mat: adjoint slot of the material, -1 untracked
                 const int m = mt[L.layer].mat;
                 if (W.inside) {
                     adj_sc(A, 0, m, P.nmat) += d;
@@ -754,7 +756,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 // albedo test (src/kernelsMod.f90:287-297)
                 const DevMat m = mt[L.layer];
                 if (rnd(L, P) < m.albedo) {
-                    if (ADJ) {
+                    if (ADJ && m.mat >= 0) {
                         adj_sc(A, 2, m.mat, P.nmat) += 1.f;
                         adj_sc(A, 3, m.mat, P.nmat) += adjC;
                     }

@@ -3,6 +3,7 @@
 // host-side output helpers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -68,15 +69,15 @@ __global__ void k_sum_f64(const double* __restrict__ p, size_t n, double* out) {
 // program is uniform across the grid).
 __global__ void k_evaluate(const DevNode* __restrict__ gnodes, DevRegion reg, long long n,
                            const float* __restrict__ pts, float* __restrict__ dist,
-                           float* __restrict__ nrm) {
+                           float* __restrict__ nrm, const float* __restrict__ accel) {
     CDevNode* nodes = (CDevNode*)gnodes;
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-    if (dist) dist[i] = region_eval<float>(nodes, reg.first, reg.count, V3<float>{x, y, z});
+    if (dist) dist[i] = region_eval<float>(nodes, reg.first, reg.count, V3<float>{x, y, z}, accel);
     if (nrm) {
         V3<Dual> p{Dual{x, 1.f, 0.f, 0.f}, Dual{y, 0.f, 1.f, 0.f}, Dual{z, 0.f, 0.f, 1.f}};
-        Dual g = region_eval<Dual>(nodes, reg.first, reg.count, p);
+        Dual g = region_eval<Dual>(nodes, reg.first, reg.count, p, accel);
         float inv = 1.0f / sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
         nrm[3 * i] = g.x * inv;
         nrm[3 * i + 1] = g.y * inv;
@@ -123,6 +124,8 @@ struct smcrt_scene {
     float circ_t[12];
     int circ_swap;
     float rdir[3];
+    // polygon accelerators (floats, see polygon_eval in smcrt_device.h)
+    std::vector<float> accel;
     // detectors (prepared for the device) and the offset of each one's bins
     std::vector<smcrt_detector> dets;
     std::vector<DevDet> ddets;
@@ -138,8 +141,119 @@ struct smcrt_scene {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
-        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold, d_det, d_det_counts;
+        d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold, d_det, d_det_counts, d_accel,
+        d_adj_mats, d_adj_mus;
 };
+
+// ---- polygon accelerator (host build) ------------------------------------------------------
+static double seg_dist(double px, double py, double ax, double ay, double ex, double ey) {
+    const double ee = ex * ex + ey * ey;
+    double t = ((px - ax) * ex + (py - ay) * ey) / ee;
+    t = t < 0 ? 0 : (t > 1 ? 1 : t);
+    const double bx = px - ax - t * ex, by = py - ay - t * ey;
+    return std::sqrt(bx * bx + by * by);
+}
+
+// even-odd crossing test of a horizontal ray from (px, py) to +x
+static bool point_in_polygon(const std::vector<double>& X, const std::vector<double>& Y, double px, double py) {
+    bool in = false;
+    const size_t n = X.size();
+    for (size_t i = 0, j = n - 1; i < n; j = i++) {
+        if ((Y[i] > py) != (Y[j] > py) && px < (X[j] - X[i]) * (py - Y[i]) / (Y[j] - Y[i]) + X[i]) in = !in;
+    }
+    return in;
+}
+
+// Build the accelerator of one polygon (n vertices at xy) and append it to A; returns its base.
+// Grid cells are about one mean edge length (8..512 cells across the polygon).
+static int build_polygon_accel(const double* xy, int n, std::vector<float>& A) {
+    std::vector<double> X(n), Y(n);
+    for (int i = 0; i < n; ++i) { X[i] = xy[2 * i]; Y[i] = xy[2 * i + 1]; }
+    double area2 = 0;
+    for (int i = 0; i < n; ++i) area2 += X[i] * Y[(i + 1) % n] - X[(i + 1) % n] * Y[i];
+    if (area2 < 0) { std::reverse(X.begin(), X.end()); std::reverse(Y.begin(), Y.end()); }
+    std::vector<double> EX(n), EY(n), NX(n), NY(n);
+    double xmin = X[0], xmax = X[0], ymin = Y[0], ymax = Y[0], perim = 0;
+    for (int i = 0; i < n; ++i) {
+        EX[i] = X[(i + 1) % n] - X[i];
+        EY[i] = Y[(i + 1) % n] - Y[i];
+        const double l = std::sqrt(EX[i] * EX[i] + EY[i] * EY[i]);
+        perim += l;
+        NX[i] = EY[i] / l;  // outward normal of a counter-clockwise edge
+        NY[i] = -EX[i] / l;
+        xmin = std::fmin(xmin, X[i]); xmax = std::fmax(xmax, X[i]);
+        ymin = std::fmin(ymin, Y[i]); ymax = std::fmax(ymax, Y[i]);
+    }
+    const double W = std::fmax(xmax - xmin, ymax - ymin);
+    const double cell = std::fmax(W / 512.0, std::fmin(W / 8.0, perim / n));
+    const double m = 2.0 * cell;
+    const double gx0 = xmin - m, gy0 = ymin - m;
+    const int nx = (int)std::ceil((xmax + m - gx0) / cell), ny = (int)std::ceil((ymax + m - gy0) / cell);
+    const double r = cell * std::sqrt(0.5);
+    const double slack = 1e-5 * std::fmax(W, 1.0);
+    const int base = (int)A.size();
+    const int eoff = base + 16, coff = eoff + 12 * n, loff = coff + 4 * nx * ny;
+    A.resize(loff, 0.f);
+    float* H = A.data() + base;
+    H[0] = (float)nx; H[1] = (float)ny; H[2] = (float)gx0; H[3] = (float)gy0;
+    H[4] = (float)(1.0 / cell); H[5] = (float)cell;
+    H[6] = (float)(gx0 + nx * cell); H[7] = (float)(gy0 + ny * cell);
+    H[8] = (float)m; H[9] = (float)eoff; H[10] = (float)coff; H[11] = (float)loff; H[12] = (float)n;
+    for (int i = 0; i < n; ++i) {
+        float* E = A.data() + eoff + 12 * i;
+        const int ip = (i + n - 1) % n, in = (i + 1) % n;
+        double n0x = NX[ip] + NX[i], n0y = NY[ip] + NY[i];  // vertex pseudo-normals (bisectors)
+        double n1x = NX[i] + NX[in], n1y = NY[i] + NY[in];
+        double l0 = std::sqrt(n0x * n0x + n0y * n0y), l1 = std::sqrt(n1x * n1x + n1y * n1y);
+        if (l0 < 1e-12) { n0x = NX[i]; n0y = NY[i]; l0 = 1; }
+        if (l1 < 1e-12) { n1x = NX[i]; n1y = NY[i]; l1 = 1; }
+        E[0] = (float)X[i]; E[1] = (float)Y[i]; E[2] = (float)EX[i]; E[3] = (float)EY[i];
+        E[4] = (float)(1.0 / (EX[i] * EX[i] + EY[i] * EY[i]));
+        E[5] = (float)(n0x / l0); E[6] = (float)(n0y / l0); E[7] = (float)(n1x / l1); E[8] = (float)(n1y / l1);
+    }
+    std::vector<double> dist(n);
+    std::vector<float> lists;
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            const double cx = gx0 + (ix + 0.5) * cell, cy = gy0 + (iy + 0.5) * cell;
+            double D = 1e300;
+            for (int i = 0; i < n; ++i) {
+                dist[i] = seg_dist(cx, cy, X[i], Y[i], EX[i], EY[i]);
+                D = std::fmin(D, dist[i]);
+            }
+            const bool in = point_in_polygon(X, Y, cx, cy);
+            float* Cc = A.data() + coff + 4 * (ix + nx * iy);
+            if (D > 2.0 * r + slack) {  // far: no edge within the cell
+                Cc[0] = 0.f;
+                Cc[1] = 0.f;
+                Cc[2] = (float)(in ? -D : D);
+            } else {
+                Cc[0] = (float)(loff + lists.size());
+                int cnt = 0;
+                for (int i = 0; i < n; ++i)
+                    if (dist[i] <= D + 2.0 * r + slack) {
+                        lists.push_back((float)(eoff + 12 * i));
+                        ++cnt;
+                    }
+                Cc[1] = (float)cnt;
+                Cc[2] = (float)(in ? -D : D);
+            }
+        }
+    A.insert(A.end(), lists.begin(), lists.end());
+    // every index the device follows must stay inside this polygon's records
+    const size_t end = A.size();
+    for (int c = 0; c < nx * ny; ++c) {
+        const float* Cc = A.data() + coff + 4 * c;
+        const size_t st = (size_t)Cc[0], cnt = (size_t)Cc[1];
+        if (cnt == 0) continue;
+        if (st < (size_t)loff || st + cnt > end) return -1;
+        for (size_t k = 0; k < cnt; ++k) {
+            const size_t e = (size_t)A[st + k];
+            if (e < (size_t)eoff || e + 12 > (size_t)coff || (e - eoff) % 12 != 0) return -1;
+        }
+    }
+    return base;
+}
 
 static bool finite_all(const double* v, int n);
 
@@ -290,7 +404,7 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
             const smcrt_node& n = d->nodes[R.first_node + i];
             bool ok_kind = false;
             switch (n.op) {
-            case SMCRT_OP_PRIM: ok_kind = n.kind >= SMCRT_SPHERE && n.kind <= SMCRT_PLANE; break;
+            case SMCRT_OP_PRIM: ok_kind = n.kind >= SMCRT_SPHERE && n.kind <= SMCRT_POLYGON; break;
             case SMCRT_OP_CSG: ok_kind = n.kind >= SMCRT_UNION && n.kind <= SMCRT_INTERSECTION; break;
             case SMCRT_OP_MOD_BEGIN:
             case SMCRT_OP_MOD_END: ok_kind = n.kind >= SMCRT_EXTRUDE && n.kind <= SMCRT_BEND; break;
@@ -330,6 +444,37 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
             dn.kind = n.kind;
             dn.k = (float)n.k;
             for (int j = 0; j < 8; ++j) dn.p[j] = (float)n.p[j];
+            if (n.op == SMCRT_OP_PRIM && n.kind == SMCRT_POLYGON) {
+                const double f = n.p[0], c = n.p[1];
+                if (!(f >= 0 && c >= 3 && f == std::floor(f) && c == std::floor(c) && f + c <= d->n_poly_vertices) ||
+                    !d->poly_vertices) {
+                    delete s;
+                    return fail(SMCRT_EINVAL, "region %d node %d: polygon vertices [%g, %g) outside the %d given", r, i,
+                                f, f + c, d->n_poly_vertices);
+                }
+                const double* xy = d->poly_vertices + 2 * (size_t)f;
+                for (int v = 0; v < (int)c; ++v) {
+                    const double ex = xy[2 * ((v + 1) % (int)c)] - xy[2 * v], ey = xy[2 * ((v + 1) % (int)c) + 1] - xy[2 * v + 1];
+                    if (!std::isfinite(xy[2 * v]) || !std::isfinite(xy[2 * v + 1]) || !(ex * ex + ey * ey > 0)) {
+                        delete s;
+                        return fail(SMCRT_EINVAL, "region %d node %d: polygon vertex %d non-finite or repeated", r, i, v);
+                    }
+                }
+                if (s->accel.size() + 16 + 12 * c > (1u << 24)) {
+                    delete s;
+                    return fail(SMCRT_ERANGE, "polygon accelerators exceed 2^24 floats");
+                }
+                const int pb = build_polygon_accel(xy, (int)c, s->accel);
+                if (pb < 0) {
+                    delete s;
+                    return fail(SMCRT_EINVAL, "region %d node %d: polygon accelerator failed its index check", r, i);
+                }
+                dn.p[0] = (float)pb;
+                if (s->accel.size() > (1u << 24)) {
+                    delete s;
+                    return fail(SMCRT_ERANGE, "polygon accelerators exceed 2^24 floats");
+                }
+            }
             bool ident = true;
             static const double I12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
             for (int j = 0; j < 12; ++j) {
@@ -427,7 +572,7 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
         DevBuf* bufs[] = {&s->d_nodes, &s->d_regions, &s->d_mats, &s->d_region_mat, &s->d_mat_mus, &s->d_work,
                           &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights,
                           &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum, &s->d_cold, &s->d_det,
-                          &s->d_det_counts};
+                          &s->d_det_counts, &s->d_accel, &s->d_adj_mats, &s->d_adj_mus};
         for (DevBuf* b : bufs) b->release();
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -479,6 +624,10 @@ static int ensure_device(smcrt_scene* s) {
         HIPCHECK(hipMemcpy(s->d_nodes.p, s->nodes.data(), s->nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(s->d_regions.p, s->regions.data(), s->regions.size() * sizeof(DevRegion), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(s->d_region_mat.p, s->region_mat.data(), s->region_mat.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!s->accel.empty()) {
+            if ((rc = s->d_accel.ensure(s->accel.size() * sizeof(float)))) return rc;
+            HIPCHECK(hipMemcpy(s->d_accel.p, s->accel.data(), s->accel.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
         if (!s->ddets.empty()) {
             if ((rc = s->d_det.ensure(s->ddets.size() * sizeof(DevDet)))) return rc;
             if ((rc = s->d_det_counts.ensure(s->det_bins * sizeof(unsigned long long)))) return rc;
@@ -506,6 +655,7 @@ static void fill_common(const smcrt_scene* s, KParams& P, KCold& Cc) {
     memset(&Cc, 0, sizeof Cc);
     P.nodes = s->d_nodes.as<DevNode>();
     P.regions = s->d_regions.as<DevRegion>();
+    P.accel = s->accel.empty() ? nullptr : s->d_accel.as<float>();
     P.mats = s->d_mats.as<DevMat>();
     P.nreg = (int)s->regions.size();
     P.nmat = (int)s->mats.size();
@@ -538,7 +688,8 @@ struct LaunchShape {
 
 // Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
 // material tables of n_sets sets, plus the adjoint's per-lane scores and gradients).
-static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0) {
+static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0,
+                              int adj_slots = 0) {
     LaunchShape L;
     L.block = (o && o->block_size > 0) ? o->block_size : 256;
     int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
@@ -551,7 +702,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (L.block < 64) L.block = 64;
     if (L.block > 256) L.block = 256;
     if (adj) {
-        const size_t per_lane = (size_t)48 * s->mats.size();
+        const size_t per_lane = (size_t)48 * (size_t)adj_slots;
         while (L.block > 64 && mats + per_lane * L.block > 65536) L.block /= 2;
         L.lds = mats + per_lane * L.block;
     }
@@ -604,8 +755,13 @@ static void choose_tile(const double* g, int nx, int ny, int nz, int budget, int
     for (int a = 0; a < 3; ++a) { t0[a] = lo[a]; tn[a] = hi[a] - lo[a]; }
 }
 
+struct AdjTables {
+    const DevMat* mats;    // region-indexed table whose `mat` is the adjoint slot (-1: untracked)
+    const float* mat_mus;  // mus per slot
+    int nslots;
+};
 static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
-                      hipStream_t stream, float* kernel_ms);
+                      hipStream_t stream, float* kernel_ms, const AdjTables* at = nullptr);
 
 // Pick the LDS tally tile from a pilot run of the scene (once per scene).
 static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stream) {
@@ -634,7 +790,7 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
 
 // Launch the packet kernel for one scene (one set).  d_jmean may be null (no tally).
 static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
-                      hipStream_t stream, float* kernel_ms) {
+                      hipStream_t stream, float* kernel_ms, const AdjTables* at) {
     if (o->n_packets == 0) {
         *kernel_ms = 0.f;
         return SMCRT_OK;
@@ -648,6 +804,11 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     KParams P;
     KCold Cc;
     fill_common(s, P, Cc);
+    if (adj && at) {
+        P.mats = at->mats;
+        P.nmat = at->nslots;
+        Cc.mat_mus = at->mat_mus;
+    }
     P.jmean = d_jmean;
     P.grid_stride = 0;
     P.seed_lo = (uint32_t)o->seed;
@@ -677,7 +838,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[a]; P.tn[a] = s->tile_tn[a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }
-    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox);
+    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat);
     P.chunk = sh.chunk;
     P.dda_cap = sh.walk_cap;
     P.refill_min = sh.refill_min;
@@ -700,7 +861,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     k_fill_u64<<<1, 64, 0, stream>>>(P.work, 8);
     k_fill_u64<<<1, 64, 0, stream>>>(Cc.stats, NOUT + NEV);
     k_fill_f64<<<1, 64, 0, stream>>>(Cc.moments, 24);
-    if (adj) k_fill_f64<<<1, 256, 0, stream>>>(Cc.grad, 4 * s->mats.size() + 1);
+    if (adj) k_fill_f64<<<1, 256, 0, stream>>>(Cc.grad, 4 * (size_t)P.nmat + 1);
     HIPCHECK(hipEventRecord(s->ev0, stream));
     if (adj) hipLaunchKernelGGL(packet_kernel<true>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
     else hipLaunchKernelGGL(packet_kernel<false>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
@@ -823,7 +984,7 @@ extern "C" int smcrt_evaluate(smcrt_scene* s, int32_t region, int64_t n, const f
     float* dn = normals ? s->d_out.as<float>() + n : nullptr;
     const DevRegion reg = s->regions[region];
     hipLaunchKernelGGL(k_evaluate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, s->d_nodes.as<DevNode>(),
-                       reg, (long long)n, s->d_pts.as<float>(), dd, dn);
+                       reg, (long long)n, s->d_pts.as<float>(), dd, dn, s->accel.empty() ? nullptr : s->d_accel.as<float>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s->stream));
     if (dist) HIPCHECK(hipMemcpy(dist, dd, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
@@ -834,13 +995,38 @@ extern "C" int smcrt_evaluate(smcrt_scene* s, int32_t region, int64_t n, const f
 extern "C" int smcrt_grad(smcrt_scene* s, const smcrt_run_opts* o, const float* weights, double* jmean,
                           smcrt_grad_out* out) {
     if (!s || !o || !out) return fail(SMCRT_EINVAL, "null scene, options or output");
-    if (s->mats.size() > 16) return fail(SMCRT_ERANGE, "the adjoint supports at most 16 materials (scene has %zu)",
-                                         s->mats.size());
+    const size_t nm = s->mats.size();
+    // adjoint slots: the materials selected by the mask (all when it is empty), at most 16
+    const uint64_t mask = ((uint64_t)o->grad_mask_hi << 32) | o->grad_mask_lo;
+    std::vector<int> slot(nm, -1), slot_mat;
+    for (size_t m = 0; m < nm; ++m)
+        if (!mask || ((mask >> m) & 1u)) {
+            slot[m] = (int)slot_mat.size();
+            slot_mat.push_back((int)m);
+        }
+    if (mask >> nm) return fail(SMCRT_EINVAL, "grad mask selects materials beyond the scene's %zu", nm);
+    if (slot_mat.size() > 16)
+        return fail(SMCRT_ERANGE, "the adjoint tracks at most 16 materials (%zu selected; use grad_mask)", slot_mat.size());
     int rc = ensure_device(s);
     if (rc) return rc;
     const size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
-    const size_t nm = s->mats.size();
-    if ((rc = s->d_grad.ensure((4 * nm + 1) * sizeof(double)))) return rc;
+    const size_t ns = slot_mat.size();
+    if ((rc = s->d_grad.ensure((4 * ns + 1) * sizeof(double)))) return rc;
+    AdjTables at;
+    {
+        std::vector<DevMat> dm(s->regions.size());
+        region_mats(s, s->mats.data(), dm.data());
+        for (DevMat& x : dm) x.mat = slot[x.mat];
+        std::vector<float> mus(ns > 0 ? ns : 1, 0.f);
+        for (size_t k = 0; k < ns; ++k) mus[k] = (float)s->mats[slot_mat[k]].mus;
+        if ((rc = s->d_adj_mats.ensure(dm.size() * sizeof(DevMat)))) return rc;
+        if ((rc = s->d_adj_mus.ensure(mus.size() * sizeof(float)))) return rc;
+        HIPCHECK(hipMemcpy(s->d_adj_mats.p, dm.data(), dm.size() * sizeof(DevMat), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(s->d_adj_mus.p, mus.data(), mus.size() * sizeof(float), hipMemcpyHostToDevice));
+        at.mats = s->d_adj_mats.as<DevMat>();
+        at.mat_mus = s->d_adj_mus.as<float>();
+        at.nslots = (int)ns;
+    }
     const float* dw = nullptr;
     if (weights) {
         if ((rc = s->d_weights.ensure(nv * sizeof(float)))) return rc;
@@ -854,16 +1040,16 @@ extern "C" int smcrt_grad(smcrt_scene* s, const smcrt_run_opts* o, const float* 
         k_fill_f64<<<(unsigned)fill_blocks(nv), 256, 0, s->stream>>>(dj, nv);
     }
     float ms = 0.f;
-    if ((rc = launch_run(s, o, dj, true, dw, s->stream, &ms))) return rc;
+    if ((rc = launch_run(s, o, dj, true, dw, s->stream, &ms, &at))) return rc;
     if ((rc = read_stats(s, &out->stats, ms))) return rc;
-    std::vector<double> g(4 * nm + 1);
+    std::vector<double> g(4 * ns + 1);
     HIPCHECK(hipMemcpy(g.data(), s->d_grad.p, g.size() * sizeof(double), hipMemcpyDeviceToHost));
     const double N = (double)(o->n_packets ? o->n_packets : 1);
     double* dst[4] = {out->d_r_dmua, out->d_r_dmus, out->d_w_dmua, out->d_w_dmus};
     for (int w = 0; w < 4; ++w)
         if (dst[w])
-            for (size_t m = 0; m < nm; ++m) dst[w][m] = g[w * nm + m] / N;
-    out->w_tally = g[4 * nm] / N;
+            for (size_t m = 0; m < nm; ++m) dst[w][m] = slot[m] >= 0 ? g[w * ns + slot[m]] / N : 0.0;
+    out->w_tally = g[4 * ns] / N;
     out->r_d = (double)out->stats.escaped_r / N;
     if (jmean) {
         HIPCHECK(hipMemcpy(jmean, dj, nv * sizeof(double), hipMemcpyDeviceToHost));

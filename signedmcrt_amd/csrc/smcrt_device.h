@@ -175,10 +175,70 @@ template <class T> SMCRT_DEV T vlen(const V3<T>& a) { return g_sqrt(vdot(a, a));
 template <class T> SMCRT_DEV V3<T> vmax0(const V3<T>& a) { return V3<T>{g_max0(a.x), g_max0(a.y), g_max0(a.z)}; }
 
 // ------------------------------------------------------------------------------------------
+// Closed polygon in the xy plane through its uniform-grid accelerator (built on the host by
+// smcrt_scene_create).  Accelerator layout, floats from `base`:
+//   header[16]: nx, ny, gx0, gy0, 1/cell, cell, gx1, gy1, margin, edges, cells, lists, n_edges
+//   edges[12 each]: vx, vy, ex, ey, 1/|e|^2, start-vertex pseudo-normal (2), end-vertex
+//                   pseudo-normal (2), pad
+//   cells[4 each]: list start, list length (0: far cell), signed centre distance, pad
+//   lists: absolute float index of each candidate edge record
+// A near cell lists every edge that can be nearest anywhere in the cell (distance from the
+// centre <= D_c + 2r, r the half diagonal), so its distance is exact; its sign comes from the
+// nearest feature (edge side, or the vertex pseudo-normal when the nearest point is a vertex).
+// A far cell (D_c > 2r, no edge inside) returns sign * (D_c - |q - c|); outside the grid the
+// distance to the grid rectangle plus the margin — both lower bounds, which keep sphere
+// tracing safe and are only used away from the surface.
+// ------------------------------------------------------------------------------------------
+template <class T>
+SMCRT_DEV T polygon_eval(const float* __restrict__ A, int base, const V3<T>& p) {
+    const float* H = A + base;
+    const float gx0 = H[2], gy0 = H[3], inv = H[4], cell = H[5], gx1 = H[6], gy1 = H[7];
+    const float qx = val(p.x), qy = val(p.y);
+    if (!(qx >= gx0 && qx < gx1 && qy >= gy0 && qy < gy1)) {
+        T dx = g_max0(g_max(gx0 - p.x, p.x - gx1));
+        T dy = g_max0(g_max(gy0 - p.y, p.y - gy1));
+        return g_sqrt(dx * dx + dy * dy) + H[8];
+    }
+    const int nx = (int)H[0], ny = (int)H[1];
+    int ix = (int)((qx - gx0) * inv), iy = (int)((qy - gy0) * inv);
+    ix = ix < nx - 1 ? ix : nx - 1;
+    iy = iy < ny - 1 ? iy : ny - 1;
+    const float* Cc = A + (int)H[10] + 4 * (ix + nx * iy);
+    const int start = (int)Cc[0], cnt = (int)Cc[1];
+    if (cnt == 0) {
+        const float sD = Cc[2];
+        T ddx = p.x - (gx0 + ((float)ix + 0.5f) * cell), ddy = p.y - (gy0 + ((float)iy + 0.5f) * cell);
+        T r = fabsf(sD) - g_sqrt(ddx * ddx + ddy * ddy);
+        return g_neg_if(r, sD < 0.f);
+    }
+    T best = g_const<T>(3.0e38f);
+    float bestv = 3.0e38f;
+    bool inside = false;
+    for (int k = 0; k < cnt; ++k) {
+        const float* E = A + (int)A[start + k];
+        const float vx = E[0], vy = E[1], ex = E[2], ey = E[3];
+        T wx = p.x - vx, wy = p.y - vy;
+        T t = g_clamp01((wx * ex + wy * ey) * E[4]);
+        T bx = wx - t * ex, by = wy - t * ey;
+        T d2 = bx * bx + by * by;
+        const float d2v = val(d2);
+        if (d2v < bestv) {
+            bestv = d2v;
+            best = d2;
+            const float tv = val(t), wxv = val(wx), wyv = val(wy);
+            if (tv <= 0.f) inside = wxv * E[5] + wyv * E[6] < 0.f;
+            else if (tv >= 1.f) inside = (wxv - ex) * E[7] + (wyv - ey) * E[8] < 0.f;
+            else inside = ex * wyv - ey * wxv > 0.f;  // left of a counter-clockwise edge
+        }
+    }
+    return g_neg_if(g_sqrt(best), inside);
+}
+
+// ------------------------------------------------------------------------------------------
 // SDF primitives (src/sdfs/sdfs.f90:494-735)
 // ------------------------------------------------------------------------------------------
 template <class T>
-SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p) {
+SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, const float* __restrict__ accel) {
     switch (kind) {
     case 1: {  // sphere
         return g_sqrt(vdot(p, p)) - P[0];
@@ -258,6 +318,8 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p) {
     case 10: {  // plane(a)
         return p.x * P[0] + p.y * P[1] + p.z * P[2];
     }
+    case 11:  // polygon (P[0]: accelerator offset)
+        return polygon_eval<T>(accel, (int)P[0], p);
     }
     return g_const<T>(0.f);
 }
@@ -331,7 +393,7 @@ SMCRT_DEV V3<T> xform(const float* __restrict__ t, const V3<T>& p) {
 // so the opcode dispatch is a scalar branch and parameters come from scalar loads.  The value
 // and position stacks are shift registers (no dynamically indexed register arrays).
 template <class T>
-SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0) {
+SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, const float* __restrict__ accel) {
     T v0 = g_const<T>(0.f), v1 = v0, v2 = v0, v3 = v0;
     V3<T> q0 = p0, q1 = p0, q2 = p0;  // q0 = top of the position stack
     for (int i = 0; i < count; ++i) {
@@ -348,7 +410,7 @@ SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0) 
                 for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
                 q = xform<T>(tt, q0);
             }
-            T d = prim_eval<T>(kind, pp, q);
+            T d = prim_eval<T>(kind, pp, q, accel);
             v3 = v2; v2 = v1; v1 = v0; v0 = d;
         } else if (op == 2) {  // CSG: d1 = v1, d2 = v0
             T d = csg_eval<T>(kind, v1, v0, n->k);

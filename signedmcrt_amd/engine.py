@@ -167,11 +167,20 @@ class Engine:
         A.check(self._lib.smcrt_scene_set_materials(self.handle, arr, len(self.scene.materials)))
 
     def grad(self, n: int, seed: int = 123456789, first_packet: int = 0, weights=None,
-             tally: bool = False, **shape) -> GradResult:
+             tally: bool = False, materials=None, **shape) -> GradResult:
+        """pMC adjoint.  `materials`: indices of the materials to differentiate (default all;
+        at most 16 — the others' derivatives come back as 0)."""
         g = self.scene.grid
         nm = self.scene.n_materials
         flags, shape = self._shape(0, shape)
         o = run_opts(n, seed, first_packet, flags, **shape)
+        if materials is not None:
+            mask = 0
+            for m in materials:
+                if not 0 <= int(m) < nm:
+                    raise ValueError(f"material {m} out of range")
+                mask |= 1 << int(m)
+            o.grad_mask_lo, o.grad_mask_hi = mask & 0xFFFFFFFF, mask >> 32
         out = A.GradOut()
         arrs = [np.zeros(nm) for _ in range(4)]
         out.d_r_dmua, out.d_r_dmus, out.d_w_dmua, out.d_w_dmus = (_dptr(a) for a in arrs)

@@ -8,7 +8,7 @@ module smcrt
     use, intrinsic :: iso_c_binding
     implicit none
 
-    integer(c_int), parameter :: SMCRT_ABI = 2  !< SMCRT_ABI_VERSION of include/smcrt.h
+    integer(c_int), parameter :: SMCRT_ABI = 3  !< SMCRT_ABI_VERSION of include/smcrt.h
 
     ! status codes
     integer(c_int), parameter :: SMCRT_OK = 0, SMCRT_EINVAL = -1, SMCRT_ENOMEM = -2, &
@@ -19,7 +19,8 @@ module smcrt
     ! primitives (src/sdfs/sdfs.f90:494-735)
     integer(c_int32_t), parameter :: SMCRT_SPHERE = 1, SMCRT_BOX = 2, SMCRT_TORUS = 3, &
                                      SMCRT_CYLINDER = 4, SMCRT_TRIPRISM = 5, SMCRT_SEGMENT = 6, &
-                                     SMCRT_CAPSULE = 7, SMCRT_CONE = 8, SMCRT_EGG = 9, SMCRT_PLANE = 10
+                                     SMCRT_CAPSULE = 7, SMCRT_CONE = 8, SMCRT_EGG = 9, SMCRT_PLANE = 10, &
+                                     SMCRT_POLYGON = 11
     ! CSG operators (src/sdfs/sdfModifiers.f90:412-475)
     integer(c_int32_t), parameter :: SMCRT_UNION = 1, SMCRT_SMOOTH_UNION = 2, &
                                      SMCRT_SUBTRACTION = 3, SMCRT_INTERSECTION = 4
@@ -76,15 +77,16 @@ module smcrt
         type(c_ptr)        :: nodes = c_null_ptr, regions = c_null_ptr, materials = c_null_ptr
         type(smcrt_grid)   :: grid
         type(smcrt_source) :: source
-        integer(c_int32_t) :: n_detectors = 0, pad = 0
+        integer(c_int32_t) :: n_detectors = 0, n_poly_vertices = 0
         type(c_ptr)        :: detectors = c_null_ptr
+        type(c_ptr)        :: poly_vertices = c_null_ptr
     end type smcrt_scene_desc
 
     type, bind(C) :: smcrt_run_opts
         integer(c_int64_t) :: n_packets = 0, seed = 123456789, first_packet = 0
         integer(c_int32_t) :: flags = 0, max_bounces = 0, block_size = 0, blocks_per_cu = 0
         integer(c_int32_t) :: chunk = 0, trace_packets = 0, trace_events = 0, walk_cap = 0
-        integer(c_int32_t) :: refill_min = 0, reserved(3) = 0
+        integer(c_int32_t) :: refill_min = 0, grad_mask_lo = 0, grad_mask_hi = 0, reserved = 0
     end type smcrt_run_opts
 
     type, bind(C) :: smcrt_stats

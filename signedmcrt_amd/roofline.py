@@ -26,6 +26,9 @@ PRIM_FLOPS = {
     A.CONE: 52,
     A.EGG: 14,
     A.PLANE: 5,
+    A.POLYGON: 16,    # grid test (4 cmp), cell index (2 fma), far-cell bound (2 sub, length 4, sub):
+                      # the far-cell path only, so polygon work is a lower bound (near cells add
+                      # ~15 per candidate edge, a count the event counters do not see)
 }
 TRANSFORM_FLOPS = 18                 # 3 components x 3 fma (identity transforms are skipped)
 CSG_FLOPS = {A.UNION: 1, A.SMOOTH_UNION: 10, A.SUBTRACTION: 1, A.INTERSECTION: 1}

@@ -92,10 +92,11 @@ class Scene:
         self.materials: List[mono] = []
         nodes: List[A.Node] = []
         regions: List[A.Region] = []
+        verts: List[float] = []
         for sdf in self.array:
             m = self._material_index(sdf.optProps)
             first = len(nodes)
-            sdf.flatten(nodes)
+            sdf.flatten(nodes, verts)
             r = A.Region()
             r.first_node, r.n_nodes, r.material, r.layer = first, len(nodes) - first, m, sdf.layer
             regions.append(r)
@@ -117,6 +118,9 @@ class Scene:
         self._dets = (A.Detector * max(1, len(self.detectors)))(*[x.to_c() for x in self.detectors])
         d.n_detectors = len(self.detectors)
         d.detectors = C.cast(self._dets, C.POINTER(A.Detector))
+        self._verts = (C.c_double * max(2, len(verts)))(*verts)
+        d.n_poly_vertices = len(verts) // 2
+        d.poly_vertices = C.cast(self._verts, C.POINTER(C.c_double))
         self.desc = d
 
     def _material_index(self, m: mono) -> int:

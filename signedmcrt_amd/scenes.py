@@ -3,7 +3,8 @@
 `build(geom_name, **params)` mirrors `setup_simulation`'s dispatch on `geom_name`
 (src/setup.f90:33-52).  Reference scenes kept: `scat_test` (src/setupGeometry.f90:221-247),
 `scat_test2` (:249-274), `aptran` (:147-175), `sphere_scene` (:62-106).  Added scenes (the
-reference's registry has none of these): `slab`, `refracting_sphere`, `skin_vessels`.
+reference's registry has none of these): `slab`, `refracting_sphere`, `skin_vessels`,
+`polygon_regions`.
 Each builder returns (array, grid, source, nphotons) with the reference's defaults where the
 reference fixes them and the benchmark's sizes otherwise.
 """
@@ -14,7 +15,8 @@ from typing import Dict, List, Tuple
 import numpy as np
 
 from .scene import Scene, cart_grid, source
-from .sdf import (SDF, SmoothUnion, box, cylinder, invert, model, mono, sphere, translate)
+from .sdf import (SDF, SmoothUnion, box, cylinder, extrude, invert, model, mono, polygon, sphere,
+                  translate)
 
 
 def scat_test(tau: float = 10.0) -> Tuple[List[SDF], cart_grid, source, int]:
@@ -131,6 +133,38 @@ def skin_vessels(seed: int = 123456789, nxyz: int = 256, k: float = 0.01
     return arr, cart_grid(nxyz, nxyz, nxyz, half_xy, half_xy, zmax), src, 100000000
 
 
+def polygon_regions(seed: int = 123456789, n_side: int = 4, n_edges: int = 125, depth: float = 0.5,
+                    half: float = 2.0, nxy: int = 400, nz: int = 50, mua=None
+                    ) -> Tuple[List[SDF], cart_grid, source, int]:
+    """n_side^2 closed polygons of n_edges vertices (2000 edges for 4 x 4 x 125), one per tile of
+    an n_side x n_side tiling of [-half, half]^2, extruded to `depth`, inside a slab of background
+    tissue of the same depth, in air.  Each polygon is a star-shaped perturbed circle
+    r(t) = 0.38 tile (1 + sum_m a_m cos(m t + phase_m)), m = 2, 3, 5, a_m in 0.02-0.08.
+    Polygon k: mu_s 50/cm, g 0.85, n 1.4, mu_a drawn log-uniform 0.05-2/cm (or mua[k]); the
+    background mu_a 0.2/cm, same mu_s, g, n.  Region order: polygons, background, air.  Uniform
+    source over the top face pointing -z; the grid is the slab (nxy x nxy x nz voxels)."""
+    rng = np.random.default_rng(seed)
+    tile = 2.0 * half / n_side
+    arr: List[SDF] = []
+    k = 0
+    for iy in range(n_side):
+        for ix in range(n_side):
+            cx, cy = -half + (ix + 0.5) * tile, -half + (iy + 0.5) * tile
+            amps = rng.uniform(0.02, 0.08, 3)
+            phases = rng.uniform(0.0, 2.0 * np.pi, 3)
+            th = 2.0 * np.pi * np.arange(n_edges) / n_edges
+            r = 0.38 * tile * (1.0 + sum(a * np.cos(m * th + ph) for a, m, ph in zip(amps, (2, 3, 5), phases)))
+            verts = np.stack([cx + r * np.cos(th), cy + r * np.sin(th)], axis=1)
+            mu = float(mua[k]) if mua is not None else float(np.exp(rng.uniform(np.log(0.05), np.log(2.0))))
+            arr.append(extrude(polygon(verts, mono(50.0, mu, 0.85, 1.4), k + 1), depth / 2.0))
+            k += 1
+    arr.append(box((2 * half, 2 * half, depth), mono(50.0, 0.2, 0.85, 1.4), k + 1))
+    arr.append(box((2 * half + 2.0, 2 * half + 2.0, depth + 2.0), mono(0.0, 0.0, 0.0, 1.0), k + 2))
+    src = source("uniform", direction=(0.0, 0.0, -1.0), point1=(-half, -half, depth / 2.0 - 1e-4),
+                 point2=(2 * half, 0.0, 0.0), point3=(0.0, 2 * half, 0.0))
+    return arr, cart_grid(nxy, nxy, nz, half, half, depth / 2.0), src, 100000000
+
+
 REGISTRY: Dict[str, callable] = {
     "scat_test": scat_test,
     "scat_test2": scat_test2,
@@ -139,6 +173,7 @@ REGISTRY: Dict[str, callable] = {
     "slab": slab,
     "refracting_sphere": refracting_sphere,
     "skin_vessels": skin_vessels,
+    "polygon_regions": polygon_regions,
 }
 
 

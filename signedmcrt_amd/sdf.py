@@ -138,7 +138,8 @@ class SDF:
     layer: int
     transform: np.ndarray
 
-    def flatten(self, nodes: List[A.Node]) -> None:
+    def flatten(self, nodes: List[A.Node], verts: List[float]) -> None:
+        """Append this SDF's postfix nodes (and any polygon vertices, x y pairs) to the lists."""
         raise NotImplementedError
 
 
@@ -153,7 +154,7 @@ class _Prim(SDF):
         self.layer = int(layer)
         self.transform = identity() if transform is None else np.asarray(transform, dtype=float)
 
-    def flatten(self, nodes: List[A.Node]) -> None:
+    def flatten(self, nodes: List[A.Node], verts: List[float]) -> None:
         n = A.Node()
         n.op, n.kind, n.k = A.OP_PRIM, self.kind, 0.0
         for i, v in enumerate(self.params):
@@ -241,6 +242,26 @@ class plane(_Prim):
         super().__init__(_v3(a), optProp, layer, transform)
 
 
+class polygon(_Prim):
+    """Closed polygon in the xy plane, `vertices` = [(x, y), ...] (simple; either orientation):
+    its signed 2-D distance, z ignored — `extrude(polygon(...), h)` makes a prism of half-height
+    h.  No reference counterpart (the reference's `segment` is a 0.1-thick tube, not a region);
+    the library evaluates it through a per-polygon uniform-grid accelerator."""
+    kind = A.POLYGON
+
+    def __init__(self, vertices, optProp, layer, transform=None):
+        v = np.asarray(vertices, dtype=float)
+        if v.ndim != 2 or v.shape[1] != 2 or len(v) < 3:
+            raise ValueError("polygon needs at least 3 (x, y) vertices")
+        self.vertices = v
+        super().__init__([0.0, float(len(v))], optProp, layer, transform)
+
+    def flatten(self, nodes: List[A.Node], verts: List[float]) -> None:
+        self.params[0] = float(len(verts) // 2)
+        verts.extend(float(x) for x in self.vertices.reshape(-1))
+        super().flatten(nodes, verts)
+
+
 # ---- operators --------------------------------------------------------------------------------
 class _Op:
     def __init__(self, kind: int, name: str, fn):
@@ -280,10 +301,10 @@ class model(SDF):
         self.layer = self.array[0].layer
         self.transform = identity()
 
-    def flatten(self, nodes: List[A.Node]) -> None:
-        self.array[0].flatten(nodes)
+    def flatten(self, nodes: List[A.Node], verts: List[float]) -> None:
+        self.array[0].flatten(nodes, verts)
         for child in self.array[1:]:
-            child.flatten(nodes)
+            child.flatten(nodes, verts)
             n = A.Node()
             n.op, n.kind, n.k = A.OP_CSG, self.func.kind, self.k
             nodes.append(n)
@@ -299,10 +320,10 @@ class _Mod(SDF):
         self.layer = prim.layer
         self.transform = identity()
 
-    def flatten(self, nodes: List[A.Node]) -> None:
+    def flatten(self, nodes: List[A.Node], verts: List[float]) -> None:
         for op in (A.OP_MOD_BEGIN, None, A.OP_MOD_END):
             if op is None:
-                self.prim.flatten(nodes)
+                self.prim.flatten(nodes, verts)
                 continue
             n = A.Node()
             n.op, n.kind = op, self.kind

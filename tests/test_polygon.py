@@ -1,0 +1,192 @@
+"""Closed-polygon primitive with its uniform-grid accelerator (no reference counterpart; needed by
+the polygon-extrusion workload).
+
+Pins: the oracle's brute-force polygon distance against an independent numpy restatement
+(winding number from summed angles) and against the 2-D box closed form (CPU); on the GPU the
+accelerated distance against the brute force at random, near-edge, near-vertex and far points
+(exact near the boundary, a sign-consistent lower bound away from it), normals against the
+oracle, the whole loop on a small polygon scene with common random numbers, the adjoint with a
+material selection, and the inverse fit recovering seeded absorption coefficients.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from signedmcrt_amd import scenes
+from signedmcrt_amd.scene import Scene, cart_grid, source
+from signedmcrt_amd.sdf import box, extrude, mono, polygon
+
+M = mono(10.0, 0.5, 0.8, 1.0)
+
+
+def star(n=40, cx=0.1, cy=-0.2, r0=0.6, seed=3):
+    rng = np.random.default_rng(seed)
+    th = np.sort(rng.uniform(0, 2 * np.pi, n))
+    r = r0 * (1 + 0.25 * np.cos(3 * th) + 0.1 * rng.uniform(-1, 1, n))
+    return np.stack([cx + r * np.cos(th), cy + r * np.sin(th)], axis=1)
+
+
+def polygon_scene(verts, clockwise=False):
+    v = verts[::-1] if clockwise else verts
+    arr = [polygon(v, M, 1), box((4, 4, 4), mono(0, 0, 0, 1), 2)]
+    return Scene(arr, cart_grid(10, 10, 10, 2, 2, 2), source("point"))
+
+
+def numpy_polygon_sdf(verts, pts):
+    """Independent restatement: distance to the nearest edge; inside where the winding number
+    (sum of the signed angles subtended by the edges) is nonzero."""
+    v = np.asarray(verts, float)
+    w = np.roll(v, -1, axis=0)
+    p = np.asarray(pts, float)[:, None, :2]
+    e = w - v
+    t = np.clip(((p - v) * e).sum(-1) / (e * e).sum(-1), 0, 1)
+    d = np.sqrt((((p - v) - t[..., None] * e) ** 2).sum(-1)).min(axis=1)
+    a = v - p
+    b = w - p
+    ang = np.arctan2(a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0], (a * b).sum(-1)).sum(axis=1)
+    inside = np.abs(ang) > np.pi
+    return np.where(inside, -d, d)
+
+
+@pytest.mark.parametrize("cw", [False, True])
+def test_oracle_polygon_matches_independent_restatement(cw):
+    v = star()
+    sc = polygon_scene(v, cw)
+    rng = np.random.default_rng(0)
+    pts = np.c_[rng.uniform(-1.2, 1.2, (4000, 2)), rng.uniform(-1, 1, 4000)]
+    np.testing.assert_allclose(O.evaluate(sc, 0, pts), numpy_polygon_sdf(v, pts), atol=1e-12, rtol=1e-12)
+
+
+def test_oracle_square_polygon_is_the_2d_box():
+    sq = np.array([[-0.3, -0.2], [0.3, -0.2], [0.3, 0.2], [-0.3, 0.2]])
+    sc = Scene([polygon(sq, M, 1), extrude(polygon(sq, M, 1), 0.1), box((0.6, 0.4, 0.2), M, 1)],
+               cart_grid(4, 4, 4, 1, 1, 1), source("point"))
+    rng = np.random.default_rng(1)
+    pts = np.c_[rng.uniform(-1, 1, (2000, 2)), rng.uniform(-0.4, 0.4, 2000)]
+    q = np.abs(pts[:, :2]) - [0.3, 0.2]
+    box2d = np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(q.max(axis=1), 0)
+    np.testing.assert_allclose(O.evaluate(sc, 0, pts), box2d, atol=1e-12)
+    # the extruded square is the 3-D box
+    np.testing.assert_allclose(O.evaluate(sc, 1, pts), O.evaluate(sc, 2, pts), atol=1e-12)
+
+
+def test_polygon_validation():
+    from signedmcrt_amd import _abi as A
+    import ctypes as C
+    with pytest.raises(ValueError):
+        polygon([[0, 0], [1, 0]], M, 1)
+    bad = Scene([polygon([[0, 0], [1, 0], [1, 0], [0, 1]], M, 1)], cart_grid(4, 4, 4, 1, 1, 1), source("point"))
+    h = C.c_void_p()
+    assert A.lib().smcrt_scene_create(C.byref(bad.desc), C.byref(h)) == A.EINVAL
+    assert "repeated" in A.last_error()
+
+
+# ---- GPU ------------------------------------------------------------------------------------
+def _probe_points(v, rng, n=20000):
+    """uniform points, points just off edges and vertices, far points outside the grid"""
+    u = np.c_[rng.uniform(-1.5, 1.5, (n, 2)), rng.uniform(-1, 1, n)]
+    i = rng.integers(0, len(v), n)
+    t = rng.uniform(0, 1, n)[:, None]
+    on = v[i] + t * (np.roll(v, -1, axis=0)[i] - v[i])
+    near = np.c_[on + rng.normal(0, 1e-3, (n, 2)), np.zeros(n)]
+    vert = np.c_[v[i] + rng.normal(0, 2e-3, (n, 2)), np.zeros(n)]
+    far = np.c_[rng.uniform(-6, 6, (n, 2)), np.zeros(n)]
+    return np.concatenate([u, near, vert, far]).astype(np.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cw", [False, True])
+def test_accelerated_polygon_matches_brute_force(gpu, cw):
+    from signedmcrt_amd.engine import Engine
+    v = star(n=125, seed=5)
+    sc = polygon_scene(v, cw)
+    eng = Engine(sc)
+    pts = _probe_points(v, np.random.default_rng(2))
+    g = eng.evaluate(0, pts).astype(float)
+    o = O.evaluate(sc, 0, pts.astype(float))
+    exact = np.abs(o) < 0.02  # within a cell or two of the boundary: the accelerator is exact
+    assert exact.sum() > 20000
+    np.testing.assert_allclose(g[exact], o[exact], atol=2e-6 + 2e-6 * np.abs(o[exact]).max())
+    # everywhere: same sign away from the boundary, and never more than the true distance
+    s = np.abs(o) > 1e-5
+    assert np.all(np.sign(g[s]) == np.sign(o[s]))
+    assert np.all(np.abs(g) <= np.abs(o) + 2e-6)
+    # and a useful bound (sphere-tracing steps stay large)
+    far = np.abs(o) > 0.1
+    assert np.median(np.abs(g[far]) / np.abs(o[far])) > 0.8
+
+
+@pytest.mark.gpu
+def test_polygon_normals(gpu):
+    from signedmcrt_amd.engine import Engine
+    v = star(n=125, seed=5)
+    sc = Scene([extrude(polygon(v, M, 1), 0.3), box((4, 4, 4), mono(0, 0, 0, 1), 2)],
+               cart_grid(10, 10, 10, 2, 2, 2), source("point"))
+    rng = np.random.default_rng(4)
+    i = rng.integers(0, len(v), 3000)
+    t = rng.uniform(0.2, 0.8, 3000)[:, None]
+    on = v[i] + t * (np.roll(v, -1, axis=0)[i] - v[i])
+    pts = np.c_[on + rng.normal(0, 3e-3, (3000, 2)), rng.uniform(-0.2, 0.2, 3000)].astype(np.float32)
+    d, n = Engine(sc).evaluate(0, pts, normals=True)
+    _, no = O.evaluate(sc, 0, pts.astype(float), normals=True)
+    cos = (n * no).sum(axis=1)
+    assert np.mean(cos > 0.9999) > 0.97, np.sort(cos)[:10]
+
+
+def small_polygon_scene(mua=None):
+    arr, grid, src, _ = scenes.polygon_regions(seed=11, n_side=2, n_edges=40, half=1.0, nxy=80, nz=10, mua=mua)
+    return Scene(arr, grid, src)
+
+
+@pytest.mark.gpu
+def test_polygon_scene_common_random_numbers(gpu):
+    """Same packets on both sides: outcome counts and the fluence of the small polygon scene
+    agree with the brute-force oracle to counting noise."""
+    from signedmcrt_amd.engine import Engine
+    from stats_util import two_sample_z
+    sc = small_polygon_scene()
+    n = 40_000
+    g = Engine(sc).run(n, seed=31)
+    o = O.run(sc, n, seed=31)
+    for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface"):
+        # common random numbers: the two runs are strongly correlated, so the independent-sample
+        # z is a loose bound; a systematic difference would still exceed it
+        assert abs(two_sample_z(g.stats[k], n, o["stats"][k], n)) < 2.0, (k, g.stats[k], o["stats"][k])
+    assert g.stats["err_unreachable"] == 0 and g.stats["err_nan"] == 0
+    assert abs(g.jmean.sum() / o["jmean"].sum() - 1) < 0.01
+    assert abs(g.stats["scatters"] / o["stats"]["scatters"] - 1) < 0.01
+
+
+@pytest.mark.gpu
+def test_adjoint_material_selection(gpu):
+    """d/dmu_a of the polygons only (the selection), against the oracle's full gradient."""
+    from signedmcrt_amd.engine import Engine
+    sc = small_polygon_scene()
+    eng = Engine(sc)
+    sel = [0, 1, 2, 3]
+    B, n = 8, 20_000
+    gg = np.array([eng.grad(n, seed=5 + b, materials=sel).d_w_dmua for b in range(B)])
+    go = np.array([O.run(sc, n, seed=105 + b, grad=True, tally=False)["d_w_dmua"] for b in range(B)])
+    assert np.all(gg[:, 4:] == 0)  # background and air not selected
+    for m in sel:
+        se = np.sqrt(gg[:, m].var(ddof=1) / B + go[:, m].var(ddof=1) / B)
+        assert abs(gg[:, m].mean() - go[:, m].mean()) < 4 * se + 1e-9, (m, gg[:, m].mean(), go[:, m].mean(), se)
+        assert gg[:, m].mean() < 0  # more absorption, less fluence
+
+
+@pytest.mark.gpu
+def test_inverse_fit_recovers_absorption(gpu):
+    from signedmcrt_amd.engine import Engine
+    from signedmcrt_amd.inverse import fit_mua
+    truth = np.array([0.15, 0.6, 1.2, 0.35])
+    sc_true = small_polygon_scene(mua=truth)
+    n = 400_000
+    meas = Engine(sc_true).run(n, seed=9999).jmean / n
+    eng = Engine(small_polygon_scene(mua=[0.5, 0.5, 0.5, 0.5]))
+    hist = fit_mua(eng, meas, materials=[0, 1, 2, 3], mua0=0.5, n_packets=n, iters=40, lr=0.08, block=10)
+    print("loss", [round(x, 5) for x in hist.loss[::5]], "fit", hist.final, "truth", truth)
+    assert hist.loss[-1] < 0.3 * hist.loss[0]
+    err = np.abs(hist.final / truth - 1)
+    assert np.all(err < 0.2), (hist.final, truth)

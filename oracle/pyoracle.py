@@ -40,7 +40,7 @@ class OrcOut(C.Structure):
     _fields_ = [("jmean", _dp), ("trace", _P(C.c_float)),
                 ("d_r_dmua", _dp), ("d_r_dmus", _dp), ("d_w_dmua", _dp), ("d_w_dmus", _dp),
                 ("w_tally", C.c_double), ("stats", A.Stats), ("seconds", C.c_double),
-                ("det_counts", _P(C.c_uint64))]
+                ("det_counts", _P(C.c_uint64)), ("weight_deposit", C.c_double)]
 
 
 _lib = None
@@ -176,7 +176,8 @@ def run(scene, n_packets: int, seed: int = 123456789, first_packet: int = 0, fla
     if rc != 0:
         raise RuntimeError(f"orc_run failed ({rc})")
     res = {"jmean": jm.reshape(g.shape, order="F") if tally else None,
-           "stats": out.stats.as_dict(), "seconds": out.seconds}
+           "stats": out.stats.as_dict(), "seconds": out.seconds,
+           "weight_deposit": out.weight_deposit}
     if dets:
         res["detectors"], off = [], 0
         for dd, sz in zip(dets, sizes):

@@ -373,6 +373,7 @@ enum { END_NONE = 0, END_ABSORB, END_ESCAPE, END_SURFACE, END_BOUNCE, END_ERROR 
 
 typedef struct {
     v3 pos, dir;
+    double weight; /* weight_scatter's packet weight */
     int layer, tflag, reason, bounces;
     orc_rng rng;
     /* adjoint accumulators */
@@ -574,6 +575,7 @@ static void tauint2(const orc_scene* S, orc_thread* T, orc_packet* k, int max_bo
             int rflag = 0;
             if (rng_u(&k->rng) <= Ri) { k->dir = reflect(k->dir, N); rflag = 1; }
             else k->dir = refract(k->dir, N, n1 / n2);
+            k->weight *= Ri; /* either branch (:167) */
             tau = -log(rng_u(&k->rng)); /* draw: resampled optical depth (:168) */
             taurun = 0.0;
             if (!rflag) {
@@ -676,6 +678,7 @@ typedef struct orc_out {
     smcrt_stats stats;
     double seconds;         /* wall time of the packet loop (omp_get_wtime) */
     uint64_t* det_counts;   /* every detector's bins, concatenated (NULL: not recorded) */
+    double weight_deposit;  /* weight estimator: total weight deposited at interactions */
 } orc_out;
 
 /* ---------------------------------------------------------------------------------------
@@ -740,10 +743,9 @@ static long axis_bin(double q, int nb) {
     return i < 1 ? nb : i;
 }
 
-/* record one hit point (pos, dir, value = sqrt(x^2+y^2), layer 1-based) on detector d */
-static void det_record(const orc_det* d, v3 pos, v3 dir, int layer, uint64_t* counts) {
+/* record one hit point (pos, dir, value, layer 1-based) on detector d */
+static void det_record(const orc_det* d, v3 pos, v3 dir, double value, int layer, uint64_t* counts) {
     if (layer != d->layer) return;
-    double value = sqrt(pos.x * pos.x + pos.y * pos.y);
     long idx = -1; /* 0-based bin */
     if (d->kind == SMCRT_DET_CIRCLE) {
         double denom = dot(d->dir, dir);
@@ -806,6 +808,8 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
     int64_t N = (int64_t)o->n_packets;
     orc_det dets[SMCRT_MAX_DETECTORS];
     int ndet = out->det_counts ? d->n_detectors : 0;
+    const int wmode = (o->flags & SMCRT_FLAG_WEIGHT) != 0;
+    double wdep = 0.0;
     {
         size_t off = 0;
         for (int i = 0; i < ndet; ++i) {
@@ -815,7 +819,7 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
             off += nb;
         }
     }
-#pragma omp parallel reduction(+ : a_abs, a_r, a_t, a_ks, a_kb, a_sc, a_err, e_sw, e_re, e_no, e_st, e_cr, e_if, e_fr, e_un, e_dc, wt)
+#pragma omp parallel reduction(+ : a_abs, a_r, a_t, a_ks, a_kb, a_sc, a_err, e_sw, e_re, e_no, e_st, e_cr, e_if, e_fr, e_un, e_dc, wt, wdep)
     {
         orc_thread T;
         memset(&T, 0, sizeof T);
@@ -849,8 +853,32 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
                 k.layer = bi;
                 T.sweeps++;
             }
+            k.weight = 1.0;
             tauint2(&S, &T, &k, max_b);
             int nsc = 0;
+            while (!k.tflag && wmode) {
+                /* weight_scatter (src/kernelsMod.f90:83-111) */
+                double wa = k.weight * (1.0 - S.albedo[k.layer]);
+                k.weight -= wa;
+                int ci, cj, ck;
+                voxel_of(&S, mk(k.pos.x + S.xmax, k.pos.y + S.ymax, k.pos.z + S.zmax), &ci, &cj, &ck);
+                if (ci < 1 || cj < 1 || ck < 1) { tag_end(&k, END_ESCAPE); break; }
+                if (out->jmean) {
+                    size_t idx = (size_t)(ci - 1) + (size_t)S.nx * ((size_t)(cj - 1) + (size_t)S.ny * (size_t)(ck - 1));
+#pragma omp atomic
+                    out->jmean[idx] += wa;
+                }
+                wdep += wa;
+                scatter(&k, S.hgg[k.layer], S.g2[k.layer]);
+                nsc++;
+                a_sc++;
+                trace_put(tr, o->trace_events, &k.tc, k.pos, 2);
+                if (k.weight < 0.01) { /* THRESHOLD, CHANCE (src/constants.f90:28-30) */
+                    if (rng_u(&k.rng) < 0.1) k.weight /= 0.1; /* draw: roulette */
+                    else { tag_end(&k, END_ABSORB); break; }
+                }
+                tauint2(&S, &T, &k, max_b);
+            }
             while (!k.tflag) {
                 if (rng_u(&k.rng) < S.albedo[k.layer]) { /* draw: albedo test */
                     if (k.grad) {
@@ -885,9 +913,12 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
             case END_BOUNCE: a_kb++; trace_put(tr, o->trace_events, &k.tc, k.pos, 7); break;
             default: a_err++; break;
             }
-            /* detectors see every ended history (src/kernelsMod.f90:302-306) */
-            if (k.reason != END_ERROR)
-                for (int i = 0; i < ndet; ++i) det_record(&dets[i], k.pos, k.dir, k.layer + 1, out->det_counts);
+            /* detectors see every ended history (src/kernelsMod.f90:302-306, :141-145); the hit
+               value is sqrt(x^2+y^2), or the packet weight for weight_scatter */
+            if (k.reason != END_ERROR) {
+                const double hv = wmode ? k.weight : sqrt(k.pos.x * k.pos.x + k.pos.y * k.pos.y);
+                for (int i = 0; i < ndet; ++i) det_record(&dets[i], k.pos, k.dir, hv, k.layer + 1, out->det_counts);
+            }
             if (k.grad) {
                 wt += k.C;
                 for (int m = 0; m < nm; ++m) {
@@ -933,6 +964,7 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
     st->ev_crossings = e_cr; st->ev_interfaces = e_if; st->ev_fresnel = e_fr;
     st->err_unreachable = e_un + a_err;
     st->err_nan = e_dc;
+    out->weight_deposit = wdep;
     if (o->grad && N > 0) {
         out->w_tally = wt / (double)N;
         for (int m = 0; m < nm; ++m) {
@@ -1036,5 +1068,6 @@ void orc_detector_hits(const smcrt_detector* det, int64_t n, const double* pos, 
     for (size_t b = 0; b < nb; ++b) counts[b] = 0;
     for (int64_t i = 0; i < n; ++i)
         det_record(&d, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
-                   mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), layer[i], counts);
+                   mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                   sqrt(pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1]), layer[i], counts);
 }

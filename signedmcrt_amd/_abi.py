@@ -28,6 +28,7 @@ FLAG_MOMENTS = 1 << 0
 FLAG_END_EARLY = 1 << 1
 FLAG_NO_TALLY = 1 << 2
 FLAG_NO_LDS_TILE = 1 << 3
+FLAG_WEIGHT = 1 << 4
 
 MAX_REGIONS = 64
 MAX_NODES = 4096

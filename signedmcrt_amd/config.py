@@ -75,15 +75,18 @@ def scene_from_toml(path: str) -> Tuple[Scene, dict]:
     return Scene(arr, grid, src, cfg["detectors"]), cfg
 
 
-def run_toml(path: str, outdir: str = ".", n_packets: int = 0) -> dict:
+def run_toml(path: str, outdir: str = ".", n_packets: int = 0, estimator: str = "pathlength") -> dict:
     """Run the input file end to end on the GPU and write the reference's output files under
-    `outdir`: data/jmean/<fluence> (normalised FP32 NRRD) and data/detectors/detector_<i>.dat."""
+    `outdir`: data/jmean/<fluence> (normalised FP32 NRRD) and data/detectors/detector_<i>.dat.
+    `estimator`: "pathlength" (`pathlength_scatter`) or "weight" (`weight_scatter`)."""
     from .engine import Engine
+    if estimator not in ("pathlength", "weight"):
+        raise ValueError(f"unknown estimator {estimator!r}")
     scene, cfg = scene_from_toml(path)
     n = n_packets or cfg["source"]["nphotons"]
     eng = Engine(scene)
     t0 = time.perf_counter()
-    r = eng.run(n, seed=cfg["simulation"]["iseed"])
+    r = eng.run(n, seed=cfg["simulation"]["iseed"], flags=A.FLAG_WEIGHT if estimator == "weight" else 0)
     secs = time.perf_counter() - t0
     L = A.lib()
     g = scene.grid

@@ -115,6 +115,7 @@ struct Lane {
     float px, py, pz, dx, dy, dz;
     float opx, opy, opz;  // position before the crossing step
     float tau, taurun, dlay_old, dprev;
+    float weight;         // packet weight (the weight_scatter estimator; 1 otherwise)
     int layer, st, bounces, nsc, set, tcount;
     bool iface;           // the current flight started at an interface
     bool ghost;           // left the grid mid-flight: moves on, untallied, to the flight's end
@@ -248,7 +249,8 @@ SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, 
     return res;
 }
 
-// LDS layout for the adjoint: scores float [4][nmat][blockDim], gradients double [4][nmat][blockDim]
+// LDS layout for the adjoint: per-lane scores float [4][nmat][blockDim], then the block's
+// gradient sums double [4][nmat] (LDS FP64 atomics when a packet ends)
 struct AdjLds {
     float* sc;
     double* g;
@@ -258,8 +260,8 @@ struct AdjLds {
 SMCRT_DEV float& adj_sc(const AdjLds& A, int which, int m, int nmat) {
     return A.sc[(which * nmat + m) * A.stride + threadIdx.x];
 }
-SMCRT_DEV double& adj_g(const AdjLds& A, int which, int m, int nmat) {
-    return A.g[(which * nmat + m) * A.stride + threadIdx.x];
+SMCRT_DEV void adj_g_add(const AdjLds& A, int which, int m, int nmat, double v) {
+    atomicAdd(A.g + which * nmat + m, v);
 }
 
 // Offer an ended history to every detector: hit point = (final position, direction,
@@ -284,7 +286,9 @@ SMCRT_DEV long long cam_bin(double q, int bins) {
 
 SMCRT_DEV void record_detectors(CKCold* C, const Lane& L) {
     const double px = L.px, py = L.py, pz = L.pz, dx = L.dx, dy = L.dy, dz = L.dz;
-    const double value = sqrt(px * px + py * py);
+    // hit value: sqrt(x^2 + y^2) (pathlength_scatter, src/kernelsMod.f90:303) or the packet
+    // weight (weight_scatter, :142)
+    const double value = (C->flags & 16u) ? (double)L.weight : sqrt(px * px + py * py);
     for (int i = 0; i < C->n_det; ++i) {
         CDevDet* D = (CDevDet*)C->det + i;
         if (L.layer != D->layer0) continue;
@@ -368,8 +372,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         unsigned char* base = smem + sizeof(DevMat) * n_mats + sizeof(float) * ((tile_vox + 3) & ~3);
         A.sc = reinterpret_cast<float*>(base);
         A.g = reinterpret_cast<double*>(base + sizeof(float) * 4 * P.nmat * blockDim.x);
-        for (int w = 0; w < 4; ++w)
-            for (int m = 0; m < P.nmat; ++m) adj_g(A, w, m, P.nmat) = 0.0;
+        for (int i = threadIdx.x; i < 4 * P.nmat; i += blockDim.x) A.g[i] = 0.0;
     }
     __syncthreads();
     float adjC = 0.f;
@@ -470,6 +473,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             L.tcount = 0;
             L.iters = 0;
             L.ghost = false;
+            L.weight = 1.f;
             out[O_LAUNCHED]++;
             if (ADJ) {
                 adjC = 0.f;
@@ -624,6 +628,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 const float n1 = mt[L.layer].n, n2 = mt[nl].n;
                 float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);
                 bool rflag = rnd(L, P) <= R;  // draw: reflect or refract
+                L.weight *= R;                // on either branch (src/inttau2.f90:167)
                 if (rflag) {
                     float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
                     L.dx -= c * nx; L.dy -= c * ny; L.dz -= c * nz;
@@ -752,6 +757,38 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 } else {
                     end_kind = classify_escape(C, L);
                 }
+            } else if (W.flight_over && (C->flags & 16u)) {
+                // weight_scatter (src/kernelsMod.f90:83-111): deposit the absorbed fraction in the
+                // interaction voxel (leaving the grid there ends the packet), scatter, then
+                // Russian roulette below THRESHOLD = 0.01 with CHANCE = 0.1 (src/constants.f90)
+                const DevMat m = mt[L.layer];
+                const float wa = L.weight * (1.f - m.albedo);
+                L.weight -= wa;
+                const int vx = (int)floorf((L.px + P.xmax) * P.sx), vy = (int)floorf((L.py + P.ymax) * P.sy),
+                          vz = (int)floorf((L.pz + P.zmax) * P.sz);
+                if ((unsigned)vx >= (unsigned)P.nx || (unsigned)vy >= (unsigned)P.ny || (unsigned)vz >= (unsigned)P.nz) {
+                    end_kind = classify_escape(C, L);
+                } else {
+                    if (grid) {
+                        unsafeAtomicAdd(grid + (size_t)vx + (size_t)P.nx * ((size_t)vy + (size_t)P.ny * vz), (double)wa);
+                        ev[E_ATOMICS]++;
+                    }
+                    L.nsc++;
+                    out[O_SCATTERS]++;
+                    scatter(P, L, m.hgg, m.g2);
+                    trace_event(C, L, 2);
+                    bool alive = true;
+                    if (L.weight < 0.01f) {
+                        if (rnd(L, P) < 0.1f) L.weight *= 10.f;  // draw: roulette
+                        else { end_kind = END_ABSORB; alive = false; }
+                    }
+                    if (alive) {
+                        L.tau = f_neglog(rnd(L, P));  // draw: next flight's optical depth
+                        L.taurun = 0.f;
+                        L.iface = false;
+                        L.st = ST_START;
+                    }
+                }
             } else if (W.flight_over) {
                 // albedo test (src/kernelsMod.f90:287-297)
                 const DevMat m = mt[L.layer];
@@ -816,16 +853,17 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                 for (int m = 0; m < P.nmat; ++m) {
                     float Lm = adj_sc(A, 0, m, P.nmat), Am = adj_sc(A, 1, m, P.nmat);
                     float Km = adj_sc(A, 2, m, P.nmat), Bm = adj_sc(A, 3, m, P.nmat);
+                    if (Lm == 0.f && Am == 0.f && Km == 0.f && Bm == 0.f) continue;  // material not visited
                     float mus = C->mat_mus[m];
                     float scs = mus > 0.f ? Km / mus : 0.f;
                     if (isr) {
-                        adj_g(A, 0, m, P.nmat) += (double)(-Lm);
-                        adj_g(A, 1, m, P.nmat) += (double)(scs - Lm);
+                        adj_g_add(A, 0, m, P.nmat, (double)(-Lm));
+                        adj_g_add(A, 1, m, P.nmat, (double)(scs - Lm));
                     }
                     float lw = Lm * adjC - Am;
                     float bw = mus > 0.f ? (Km * adjC - Bm) / mus : 0.f;
-                    adj_g(A, 2, m, P.nmat) += (double)(-lw);
-                    adj_g(A, 3, m, P.nmat) += (double)(bw - lw);
+                    adj_g_add(A, 2, m, P.nmat, (double)(-lw));
+                    adj_g_add(A, 3, m, P.nmat, (double)(bw - lw));
                 }
             }
             L.st = ST_IDLE;
@@ -860,12 +898,9 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         if (lane_id == 0 && v) atomicAdd(evg + i, v);
     }
     if (ADJ) {
-        for (int w = 0; w < 4; ++w)
-            for (int m = 0; m < P.nmat; ++m) {
-                double v = adj_g(A, w, m, P.nmat);
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane_id == 0) atomicAdd(C->grad + w * P.nmat + m, v);
-            }
+        __syncthreads();  // every wave has left the packet loop: the block's sums are final
+        for (int i = threadIdx.x; i < 4 * P.nmat; i += blockDim.x)
+            if (A.g[i] != 0.0) atomicAdd(C->grad + i, A.g[i]);
         double v = adjW;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane_id == 0) atomicAdd(C->grad + 4 * P.nmat, v);

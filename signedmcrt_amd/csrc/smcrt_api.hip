@@ -702,9 +702,10 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (L.block < 64) L.block = 64;
     if (L.block > 256) L.block = 256;
     if (adj) {
-        const size_t per_lane = (size_t)48 * (size_t)adj_slots;
-        while (L.block > 64 && mats + per_lane * L.block > 65536) L.block /= 2;
-        L.lds = mats + per_lane * L.block;
+        // per lane: 4 FP32 scores per adjoint material; per block: 4 FP64 sums per material
+        const size_t per_lane = (size_t)16 * (size_t)adj_slots, per_block = (size_t)32 * (size_t)adj_slots;
+        while (L.block > 64 && mats + per_lane * L.block + per_block > 65536) L.block /= 2;
+        L.lds = mats + per_lane * L.block + per_block;
     }
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once

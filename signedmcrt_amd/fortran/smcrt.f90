@@ -35,7 +35,8 @@ module smcrt
     integer, parameter :: SMCRT_MAX_DETECTORS = 16
     ! run flags
     integer(c_int32_t), parameter :: SMCRT_FLAG_MOMENTS = 1, SMCRT_FLAG_END_EARLY = 2, &
-                                     SMCRT_FLAG_NO_TALLY = 4, SMCRT_FLAG_NO_LDS_TILE = 8
+                                     SMCRT_FLAG_NO_TALLY = 4, SMCRT_FLAG_NO_LDS_TILE = 8, &
+                                     SMCRT_FLAG_WEIGHT = 16
 
     type, bind(C) :: smcrt_node
         integer(c_int32_t) :: op = 0, kind = 0

@@ -1,8 +1,10 @@
 !> Fortran host driver: the reference's `mcpolar` program running `pathlength_scatter`
 !! (app/main.f90, src/kernelsMod.f90:165-333) with the packet loop in libsmcrt.so.
 !!
-!!     smcrt_run [input.toml]        (default scat_test.toml, as app/main.f90)
+!!     smcrt_run [input.toml] [--weight]   (default scat_test.toml, as app/main.f90)
 !!
+!! --weight runs the `weight_scatter` estimator (src/kernelsMod.f90:12-163) instead of
+!! `pathlength_scatter`.
 !! Reads the reference's TOML input (smcrt_config_load), builds the geometry named by
 !! [geometry] geom_name (smcrt_geometry), runs `nphotons` packets with seed `iseed`, prints
 !! Photons/s and the mean number of scatters, writes the normalised fluence to
@@ -32,12 +34,19 @@ program smcrt_main
     integer :: i, status
     character(len=:), allocatable :: name, meta
     character(len=32) :: buf
+    character(len=512) :: arg
+    logical :: weight
 
-    if (command_argument_count() == 0) then
-        file = "scat_test.toml"
-    else
-        call get_command_argument(1, file)
-    end if
+    file = "scat_test.toml"
+    weight = .false.
+    do i = 1, command_argument_count()
+        call get_command_argument(i, arg)
+        if (trim(arg) == "--weight") then
+            weight = .true.
+        else
+            file = arg
+        end if
+    end do
 
     status = smcrt_config_load(trim(file)//c_null_char, cfg)
     call check(status)
@@ -83,6 +92,7 @@ program smcrt_main
     if (cfg%n_detectors > 0) tal%detector_counts = c_loc(counts)
     opts%n_packets = cfg%nphotons
     opts%seed = cfg%iseed
+    if (weight) opts%flags = SMCRT_FLAG_WEIGHT
 
     call system_clock(c0, rate)
     status = smcrt_run(scene, opts, tal)

@@ -336,6 +336,12 @@ int smcrt_batch_run(smcrt_batch* batch);
 int smcrt_batch_read(smcrt_batch* batch, int32_t set, double* jmean, smcrt_stats* stats);
 int smcrt_batch_destroy(smcrt_batch* batch);
 
+/* `render_sub` (src/sdfs/sdf_base.f90:302-346): voxelise the scene on the GPU.  With
+   ns = nint(samples/2) and wid = grid half size / ns per axis, image(i,j,k) (1-based, column-
+   major, samples[0] x samples[1] x samples[2] floats) = min over every region of its signed
+   distance at ((i-ns_x) wid_x, (j-ns_y) wid_y, (k-ns_z) wid_z). */
+int smcrt_render(smcrt_scene* scene, const int32_t samples[3], float* image);
+
 /* Number of bins of detector `i` of a scene (0 if out of range). */
 int64_t smcrt_detector_bins(const smcrt_scene* scene, int32_t i);
 

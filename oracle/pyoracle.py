@@ -66,6 +66,8 @@ def lib() -> C.CDLL:
         L.orc_walk.argtypes = [_P(A.SceneDesc), _dp, _dp, C.c_double, _dp, _P(C.c_int)]
         L.orc_detector_hits.restype = None
         L.orc_detector_hits.argtypes = [_P(A.Detector), C.c_int64, _dp, _dp, _P(C.c_int), _P(C.c_uint64)]
+        L.orc_render.restype = None
+        L.orc_render.argtypes = [_P(A.SceneDesc), _P(C.c_int32), _dp]
         L.orc_uniforms.restype = None
         L.orc_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, _dp]
         _lib = L
@@ -133,6 +135,14 @@ def detector_hits(det, pos, dirs, layers):
     lib().orc_detector_hits(C.byref(dc), len(pos), _ptr(pos), _ptr(dirs), _ptr(lay, C.c_int),
                             _ptr(counts, C.c_uint64))
     return counts.reshape(det.shape, order="F")
+
+
+def render(scene, samples):
+    """`render_sub` in FP64: the scene's minimum signed distance on the render grid."""
+    s = (C.c_int32 * 3)(*[int(v) for v in samples])
+    img = np.empty(int(np.prod(samples)))
+    lib().orc_render(C.byref(scene.desc), s, _ptr(img))
+    return img.reshape(tuple(int(v) for v in samples), order="F")
 
 
 def uniforms(seed: int, pid: int, n: int):

@@ -1071,3 +1071,24 @@ void orc_detector_hits(const smcrt_detector* det, int64_t n, const double* pos, 
                    mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
                    sqrt(pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1]), layer[i], counts);
 }
+
+/* `render_sub` (src/sdfs/sdf_base.f90:302-346): ns = nint(samples/2), wid = extent/ns, image
+   (column-major) = min over every region at ((i-ns) wid, (j-ns) wid, (k-ns) wid), i 1-based */
+void orc_render(const smcrt_scene_desc* d, const int32_t* samples, double* image) {
+    double ns[3], wid[3];
+    const double ext[3] = {d->grid.xmax, d->grid.ymax, d->grid.zmax};
+    for (int a = 0; a < 3; ++a) {
+        ns[a] = (double)llround(samples[a] / 2.0);
+        wid[a] = ext[a] / ns[a];
+    }
+    const int64_t sx = samples[0], sy = samples[1], sz = samples[2];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < sz; ++k)
+        for (int64_t j = 0; j < sy; ++j)
+            for (int64_t i = 0; i < sx; ++i) {
+                v3 p = mk((i + 1 - ns[0]) * wid[0], (j + 1 - ns[1]) * wid[1], (k + 1 - ns[2]) * wid[2]);
+                double m = INFINITY;
+                for (int r = 0; r < d->n_regions; ++r) m = fmin(m, region_eval(d, r, p));
+                image[i + sx * (j + sy * k)] = m;
+            }
+}

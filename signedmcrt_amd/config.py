@@ -108,6 +108,13 @@ def run_toml(path: str, outdir: str = ".", n_packets: int = 0, estimator: str = 
         dc = d.to_c()
         A.check(L.smcrt_write_detector(os.fsencode(p), C.byref(dc), c.ctypes.data_as(C.POINTER(C.c_uint64))))
         dpaths.append(p)
-    return {"result": r, "fluence_path": fpath, "detector_paths": dpaths, "seconds": secs,
+    rpath = None
+    if cfg["output"]["render_geom"]:  # src/kernelsMod.f90:532-536
+        img = eng.render(cfg["output"]["render_size"])
+        rpath = os.path.join(outdir, "data", cfg["output"]["render"])
+        flat = np.ascontiguousarray(img.reshape(-1, order="F"))
+        A.check(L.smcrt_write_nrrd(os.fsencode(rpath), flat.ctypes.data_as(C.POINTER(C.c_float)),
+                                   *[int(v) for v in img.shape], b""))
+    return {"result": r, "fluence_path": fpath, "detector_paths": dpaths, "render_path": rpath, "seconds": secs,
             "photons_per_s": n / secs, "mean_scatters": r.stats["scatters"] / n,
             "fluence": flu.reshape(g.shape, order="F")}

@@ -85,6 +85,23 @@ __global__ void k_evaluate(const DevNode* __restrict__ gnodes, DevRegion reg, lo
     }
 }
 
+// render_sub: minimum over every region at one voxel centre per thread (regions are
+// wave-uniform: scalar program loads as in the packet kernel)
+__global__ void k_render(const DevNode* __restrict__ gnodes, const DevRegion* __restrict__ gregions, int nreg,
+                         int sx, int sy, int sz, float nsx, float nsy, float nsz, float wx, float wy, float wz,
+                         float* __restrict__ image, const float* __restrict__ accel) {
+    CDevNode* nodes = (CDevNode*)gnodes;
+    CDevRegion* regions = (CDevRegion*)gregions;
+    const long long n = (long long)sx * sy * sz;
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(v % sx), j = (int)((v / sx) % sy), k = (int)(v / ((long long)sx * sy));
+        const V3<float> p{((float)(i + 1) - nsx) * wx, ((float)(j + 1) - nsy) * wy, ((float)(k + 1) - nsz) * wz};
+        float d = __builtin_inff();
+        for (int r = 0; r < nreg; ++r) d = fminf(d, region_eval<float>(nodes, regions[r].first, regions[r].count, p, accel));
+        image[v] = d;
+    }
+}
+
 static inline size_t fill_blocks(size_t n) {
     size_t b = (n + 255) / 256;
     return b < 4096 ? (b ? b : 1) : 4096;
@@ -579,6 +596,35 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
         if (s->stream) (void)hipStreamDestroy(s->stream);
     }
     delete s;
+    return SMCRT_OK;
+}
+
+static int ensure_device(smcrt_scene* s);
+
+extern "C" int smcrt_render(smcrt_scene* s, const int32_t samples[3], float* image) {
+    if (!s || !samples || !image) return fail(SMCRT_EINVAL, "null scene, samples or image");
+    for (int a = 0; a < 3; ++a)
+        if (samples[a] < 1) return fail(SMCRT_EINVAL, "render samples must be >= 1 (axis %d: %d)", a, samples[a]);
+    const size_t nv = (size_t)samples[0] * samples[1] * samples[2];
+    if (nv > ((size_t)1 << 31)) return fail(SMCRT_ERANGE, "render grid has more than 2^31 voxels");
+    int rc = ensure_device(s);
+    if (rc) return rc;
+    if ((rc = s->d_out.ensure(nv * sizeof(float)))) return rc;
+    // ns = nint(samples / 2) (Fortran nint: half away from zero), wid = extent / ns
+    const double ext[3] = {s->grid.xmax, s->grid.ymax, s->grid.zmax};
+    float ns[3], wid[3];
+    for (int a = 0; a < 3; ++a) {
+        ns[a] = (float)std::floor(samples[a] / 2.0 + 0.5);
+        wid[a] = (float)(ext[a] / (double)ns[a]);
+    }
+    const unsigned blocks = (unsigned)std::min<size_t>((nv + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_render, dim3(blocks), dim3(256), 0, s->stream, s->d_nodes.as<DevNode>(),
+                       s->d_regions.as<DevRegion>(), (int)s->regions.size(), samples[0], samples[1], samples[2],
+                       ns[0], ns[1], ns[2], wid[0], wid[1], wid[2], s->d_out.as<float>(),
+                       s->accel.empty() ? nullptr : s->d_accel.as<float>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    HIPCHECK(hipMemcpy(image, s->d_out.p, nv * sizeof(float), hipMemcpyDeviceToHost));
     return SMCRT_OK;
 }
 

@@ -161,6 +161,14 @@ class Engine:
                                          _fptr(nn) if normals else None))
         return (d, nn) if normals else d
 
+    def render(self, samples=(200, 200, 200)) -> np.ndarray:
+        """`render_sub` (src/sdfs/sdf_base.f90:302-346): the scene's minimum signed distance on a
+        samples[0] x samples[1] x samples[2] grid (FP32, Fortran order)."""
+        s = (C.c_int32 * 3)(*[int(v) for v in samples])
+        img = np.empty(int(np.prod(samples)), dtype=np.float32)
+        A.check(self._lib.smcrt_render(self.handle, s, _fptr(img)))
+        return img.reshape(tuple(int(v) for v in samples), order="F")
+
     # -- materials / adjoint -------------------------------------------------------------------
     def set_materials(self, mus=None, mua=None) -> None:
         arr = self.scene.material_array(mus, mua)

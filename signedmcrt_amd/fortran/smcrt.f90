@@ -191,6 +191,13 @@ module smcrt
             type(c_ptr), value :: pts, dist, normals
         end function smcrt_evaluate
 
+        integer(c_int) function smcrt_render(scene, samples, image) bind(C, name="smcrt_render")
+            import :: c_int, c_ptr, c_int32_t, c_float
+            type(c_ptr), value :: scene
+            integer(c_int32_t), intent(in) :: samples(3)
+            real(c_float), intent(out) :: image(*)
+        end function smcrt_render
+
         integer(c_int64_t) function smcrt_detector_bins(scene, i) bind(C, name="smcrt_detector_bins")
             import :: c_int64_t, c_ptr, c_int32_t
             type(c_ptr), value :: scene

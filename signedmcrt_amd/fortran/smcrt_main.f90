@@ -27,7 +27,7 @@ program smcrt_main
     type(smcrt_tallies) :: tal
     type(c_ptr) :: scene
     real(c_double), allocatable, target :: jmean(:, :, :)
-    real(c_float), allocatable :: fluence(:, :, :)
+    real(c_float), allocatable :: fluence(:, :, :), image(:, :, :)
     integer(c_int64_t), allocatable, target :: counts(:)
     integer(c_int64_t) :: nb, off
     integer(int64) :: c0, c1, rate
@@ -119,6 +119,15 @@ program smcrt_main
         call check(status)
         off = off + smcrt_detector_bins(scene, int(i - 1, c_int32_t))
     end do
+    ! geometry render (src/kernelsMod.f90:532-536, render_sub in src/sdfs/sdf_base.f90:302-346)
+    if (cfg%render_geom /= 0) then
+        allocate(image(cfg%render_size(1), cfg%render_size(2), cfg%render_size(3)))
+        status = smcrt_render(scene, cfg%render_size, image)
+        call check(status)
+        status = smcrt_write_nrrd("data/"//c_string(cfg%render)//c_null_char, image, cfg%render_size(1), &
+                                  cfg%render_size(2), cfg%render_size(3), c_null_char)
+        call check(status)
+    end if
     status = smcrt_scene_destroy(scene)
 
 contains

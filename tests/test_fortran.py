@@ -76,6 +76,11 @@ def test_driver_matches_python_host(gpu, tmp_path):
         a = (fdir / "data" / "detectors" / f"detector_{i}.dat").read_text()
         b = open(out["detector_paths"][i - 1]).read()
         assert a == b, i
+    # render_geom: both hosts voxelise the scene with smcrt_render (same floats)
+    hr, dr = _read_nrrd(fdir / "data" / "geom.nrrd")
+    hq, dq = _read_nrrd(out["render_path"])
+    assert hr[3] == hq[3] == "sizes: 20 20 10"
+    np.testing.assert_array_equal(dr, dq)
     # the circle detector's file lists bins 1..nbins+1 with j*bin_width and the count
     rows = [line.split() for line in (fdir / "data" / "detectors" / "detector_1.dat").read_text().splitlines()]
     assert len(rows) == 21

@@ -234,7 +234,7 @@ SMCRT_DEV float fresnel(float ix, float iy, float iz, float nx, float ny, float 
 
 // Evaluate, per lane, the region r_lane at p (only lanes with `active` set).  Waterfall over the
 // distinct region ids in the wavefront so each evaluation runs the wave-uniform interpreter.
-template <class T>
+template <class T, uint32_t KM>
 SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane, const V3<T>& p,
                              const float* __restrict__ accel) {
     T res = g_const<T>(0.f);
@@ -243,7 +243,106 @@ SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, 
         int leader = __builtin_ctzll(pending);
         int r = __builtin_amdgcn_readlane(r_lane, leader);
         bool mine = active && r_lane == r;
-        if (mine) res = region_eval<T>(nodes, regions[r].first, regions[r].count, p, accel);
+        if (mine) res = region_eval<T, KM>(nodes, regions[r].first, regions[r].count, p, accel);
+        pending &= ~wave_ballot(mine);
+    }
+    return res;
+}
+
+// One primitive node at p: transform (unless the identity), then the primitive.
+template <class T, uint32_t KM>
+SMCRT_DEV T prim_node(CDevNode* n, const V3<T>& p, const float* __restrict__ accel) {
+    float pp[8], tt[12];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) pp[j] = n->p[j];
+    V3<T> q = p;
+    if (!n->ident) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
+        q = xform<T>(tt, p);
+    }
+    return prim_eval<T, KM>(n->kind, pp, q, accel);
+}
+
+SMCRT_DEV V3<float> dual_grad(const Dual& d) { return V3<float>{d.x, d.y, d.z}; }
+
+// Exact gradient of region r at p.  A flat fold of primitives by union or SmoothUnion (a `model`
+// of primitives, DevRegion.flat) is differentiated lazily, forward-mode through the fold:
+// running value a, gradient g; each child c_k enters as a <- op(a, c_k), g <- alpha g + beta
+// grad(c_k) with alpha, beta the partials of op, and grad(c_k) (a Dual evaluation of one
+// primitive) is computed only where beta != 0.  The walk starts at the last child that wins
+// outright (alpha = 0 there), found in a first float pass, so children that cannot shape the
+// surface at p cost one float evaluation.  Any other program runs the Dual interpreter once.
+template <uint32_t KM>
+SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, CDevRegion* R, const V3<float>& p, const float* __restrict__ accel) {
+    const int first = R->first, count = R->count, flat = R->flat;
+    const V3<Dual> pd{Dual{p.x, 1.f, 0.f, 0.f}, Dual{p.y, 0.f, 1.f, 0.f}, Dual{p.z, 0.f, 0.f, 1.f}};
+    // flat folds exist only in programs with a union or SmoothUnion
+    if constexpr (!KM_HAS(KM, KM_CSG(1) | KM_CSG(2))) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
+    if (!flat) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
+    const int nch = (count + 1) / 2;
+    const bool smooth = KM_HAS(KM, KM_CSG(2)) && flat == 2;
+    const float k = smooth ? nodes[first + 2].k : 0.f;
+    // pass 1: index of the last child that wins outright
+    float a = prim_node<float, KM>(nodes + first, p, accel);
+    int last = 0;
+    for (int j = 1; j < nch; ++j) {
+        const float c = prim_node<float, KM>(nodes + first + 2 * j - 1, p, accel);
+        const bool wins = smooth ? (a - c >= k) : (c < a);
+        last = wins ? j : last;
+        a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k) : fminf(a, c);
+    }
+    // pass 2: forward-mode from that child on
+    int li = last == 0 ? first : first + 2 * last - 1;
+    V3<float> g{0.f, 0.f, 0.f};
+    a = 0.f;
+    {
+        // the starting child's gradient (a waterfall over the lanes' distinct start children)
+        uint64_t pending = wave_ballot(true);
+        while (pending) {
+            const int lead = __builtin_ctzll(pending);
+            const int ls = __builtin_amdgcn_readlane(li, lead);
+            const bool mine = li == ls;
+            if (mine) {
+                const Dual d = prim_node<Dual, KM>(nodes + ls, pd, accel);
+                a = d.v;
+                g = dual_grad(d);
+            }
+            pending &= ~wave_ballot(mine);
+        }
+    }
+    for (int j = 1; j < nch; ++j) {
+        if (j <= last) continue;
+        CDevNode* n = nodes + first + 2 * j - 1;
+        const float c = prim_node<float, KM>(n, p, accel);
+        float alpha, beta;
+        if (smooth) {
+            const float h = fmaxf(k - fabsf(a - c), 0.f) * (1.0f / k), h2 = 0.5f * h * h;
+            alpha = c < a ? h2 : 1.f - h2;
+            beta = c < a ? 1.f - h2 : h2;
+        } else {
+            alpha = c < a ? 0.f : 1.f;
+            beta = c < a ? 1.f : 0.f;
+        }
+        V3<float> gc{0.f, 0.f, 0.f};
+        if (beta != 0.f) gc = dual_grad(prim_node<Dual, KM>(n, pd, accel));
+        g = V3<float>{alpha * g.x + beta * gc.x, alpha * g.y + beta * gc.y, alpha * g.z + beta * gc.z};
+        a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k) : fminf(a, c);
+    }
+    return g;
+}
+
+// The gradient of each lane's region (waterfall over the distinct regions of active lanes).
+template <uint32_t KM>
+SMCRT_DEV V3<float> lane_region_gradient(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane,
+                                         const V3<float>& p, const float* __restrict__ accel) {
+    V3<float> res{0.f, 0.f, 0.f};
+    uint64_t pending = wave_ballot(active);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int r = __builtin_amdgcn_readlane(r_lane, leader);
+        const bool mine = active && r_lane == r;
+        if (mine) res = region_gradient<KM>(nodes, regions + r, p, accel);
         pending &= ~wave_ballot(mine);
     }
     return res;
@@ -351,7 +450,7 @@ SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
     return true;
 }
 
-template <bool ADJ>
+template <bool ADJ, uint32_t KM>
 __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     CDevNode* const nodes = (CDevNode*)P.nodes;
@@ -492,7 +591,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         if (sweeping) {
             V3<float> p{L.px, L.py, L.pz};
             for (int r = 0; r < P.nreg; ++r) {
-                float d = region_eval<float>(nodes, regions[r].first, regions[r].count, p, P.accel);
+                float d = region_eval<float, KM>(nodes, regions[r].first, regions[r].count, p, P.accel);
                 dmin = fminf(dmin, fabsf(d));
                 smin = fminf(smin, d);
                 if (r == 0) d0 = d;
@@ -579,51 +678,38 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         }
         ev[E_IFACE] += __builtin_popcountll(wave_ballot(cross));
 
-        // Off-sweep region evaluations share one interpreter instance (single-direction duals):
-        // pass 0 = dstmp(new), region nl at the pre-crossing position; passes 1..3 = the gradient
-        // of the normal region, one seed direction per pass (exact, value + one partial).
-        float gxyz[3] = {0.f, 0.f, 0.f};
-        {
-            const uint64_t md = wave_ballot(want_dstmp);
+        // dstmp(new): region nl at the pre-crossing position (waterfall over distinct regions)
+        if (const uint64_t md = wave_ballot(want_dstmp)) {
             ev[E_REGION] += __builtin_popcountll(md);
-#pragma nounroll
-            for (int pass = md ? 0 : 1; pass < 4; ++pass) {
-                if (pass == 1 && !wave_ballot(want_normal)) break;
-                const bool active = pass == 0 ? want_dstmp : want_normal;
-                const bool p0 = pass == 0;
-                V3<D1> pd{D1{p0 ? L.opx : L.px, pass == 1 ? 1.f : 0.f}, D1{p0 ? L.opy : L.py, pass == 2 ? 1.f : 0.f},
-                          D1{p0 ? L.opz : L.pz, pass == 3 ? 1.f : 0.f}};
-                const D1 r = eval_lane_region<D1>(nodes, regions, active, p0 ? fn : use_region, pd, P.accel);
-                if (p0) {
-                    if (want_dstmp) {
-                        const float dt = r.v, dl = dlay, dn = dfirst;
-                        if (dt < 0.f) {
-                            use_region = dt > L.dlay_old ? fn : L.layer;
-                            want_normal = true;
-                        } else if (dl > 0.f && dn < 0.f) {
-                            use_region = L.layer;
-                            want_normal = true;
-                        } else if (dl > 0.f && dn > 0.f) {
-                            end_kind = classify_escape(C, L);
-                        } else {
-                            end_kind = END_ERR;  // "This should not be reached!" (:156)
-                            ev[E_UNREACH]++;
-                        }
-                    }
+            const float dt = eval_lane_region<float, KM>(nodes, regions, want_dstmp, fn, V3<float>{L.opx, L.opy, L.opz}, P.accel);
+            if (want_dstmp) {
+                const float dl = dlay, dn = dfirst;
+                if (dt < 0.f) {
+                    use_region = dt > L.dlay_old ? fn : L.layer;
+                    want_normal = true;
+                } else if (dl > 0.f && dn < 0.f) {
+                    use_region = L.layer;
+                    want_normal = true;
+                } else if (dl > 0.f && dn > 0.f) {
+                    end_kind = classify_escape(C, L);
                 } else {
-                    gxyz[0] = pass == 1 ? r.d : gxyz[0];
-                    gxyz[1] = pass == 2 ? r.d : gxyz[1];
-                    gxyz[2] = pass == 3 ? r.d : gxyz[2];
+                    end_kind = END_ERR;  // "This should not be reached!" (:156)
+                    ev[E_UNREACH]++;
                 }
             }
         }
+
+        // exact gradient of the normal's region (one forward-mode pass; flat folds lazily)
+        V3<float> gxyz{0.f, 0.f, 0.f};
+        if (wave_ballot(want_normal))
+            gxyz = lane_region_gradient<KM>(nodes, regions, want_normal, use_region, V3<float>{L.px, L.py, L.pz}, P.accel);
 
         // normal + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
         if (const uint64_t m = wave_ballot(want_normal)) {
             ev[E_NORMALS] += __builtin_popcountll(m);
             if (want_normal) {
-                float inv = f_rsqrt(gxyz[0] * gxyz[0] + gxyz[1] * gxyz[1] + gxyz[2] * gxyz[2]);
-                float nx = gxyz[0] * inv, ny = gxyz[1] * inv, nz = gxyz[2] * inv;
+                float inv = f_rsqrt(gxyz.x * gxyz.x + gxyz.y * gxyz.y + gxyz.z * gxyz.z);
+                float nx = gxyz.x * inv, ny = gxyz.y * inv, nz = gxyz.z * inv;
                 const int nl = fn;
                 const float n1 = mt[L.layer].n, n2 = mt[nl].n;
                 float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);

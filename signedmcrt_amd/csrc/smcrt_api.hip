@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -67,17 +68,18 @@ __global__ void k_sum_f64(const double* __restrict__ p, size_t n, double* out) {
 
 // SDF values / exact gradients of one region at n points (one point per thread; the region's
 // program is uniform across the grid).
-__global__ void k_evaluate(const DevNode* __restrict__ gnodes, DevRegion reg, long long n,
-                           const float* __restrict__ pts, float* __restrict__ dist,
+__global__ void k_evaluate(const DevNode* __restrict__ gnodes, const DevRegion* __restrict__ gregions, int region,
+                           long long n, const float* __restrict__ pts, float* __restrict__ dist,
                            float* __restrict__ nrm, const float* __restrict__ accel) {
     CDevNode* nodes = (CDevNode*)gnodes;
+    CDevRegion* R = (CDevRegion*)gregions + region;
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-    if (dist) dist[i] = region_eval<float>(nodes, reg.first, reg.count, V3<float>{x, y, z}, accel);
+    if (dist) dist[i] = region_eval<float>(nodes, R->first, R->count, V3<float>{x, y, z}, accel);
     if (nrm) {
-        V3<Dual> p{Dual{x, 1.f, 0.f, 0.f}, Dual{y, 0.f, 1.f, 0.f}, Dual{z, 0.f, 0.f, 1.f}};
-        Dual g = region_eval<Dual>(nodes, reg.first, reg.count, p, accel);
+        // the packet kernel's gradient (lazy through flat folds)
+        const V3<float> g = region_gradient<KM_FULL>(nodes, R, V3<float>{x, y, z}, accel);
         float inv = 1.0f / sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
         nrm[3 * i] = g.x * inv;
         nrm[3 * i + 1] = g.y * inv;
@@ -143,6 +145,8 @@ struct smcrt_scene {
     float rdir[3];
     // polygon accelerators (floats, see polygon_eval in smcrt_device.h)
     std::vector<float> accel;
+    // primitive, operator and modifier kinds the program uses (KM_* bits of smcrt_device.h)
+    uint32_t kind_mask = 0;
     // detectors (prepared for the device) and the offset of each one's bins
     std::vector<smcrt_detector> dets;
     std::vector<DevDet> ddets;
@@ -416,7 +420,7 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
             return fail(SMCRT_EINVAL, "region %d material %d out of range", r, R.material);
         }
         int nv = 0, np = 1;
-        DevRegion dr{(int)s->nodes.size(), R.n_nodes, R.material, 0};
+        DevRegion dr{(int)s->nodes.size(), R.n_nodes, R.material, 0};  // flat set below
         for (int i = 0; i < R.n_nodes; ++i) {
             const smcrt_node& n = d->nodes[R.first_node + i];
             bool ok_kind = false;
@@ -431,6 +435,7 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
                 delete s;
                 return fail(SMCRT_EINVAL, "region %d node %d: bad op/kind (%d, %d)", r, i, n.op, n.kind);
             }
+            s->kind_mask |= n.op == SMCRT_OP_PRIM ? KM_PRIM(n.kind) : (n.op == SMCRT_OP_CSG ? KM_CSG(n.kind) : KM_MOD(n.kind));
             if (!finite_all(n.p, 8) || !finite_all(n.t, 12) || !std::isfinite(n.k)) {
                 delete s;
                 return fail(SMCRT_EINVAL, "region %d node %d: non-finite parameter", r, i);
@@ -504,6 +509,18 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
         if (nv != 1 || np != 1) {
             delete s;
             return fail(SMCRT_EINVAL, "region %d: program leaves %d distances and %d positions (want 1, 1)", r, nv, np);
+        }
+        // flat fold of primitives (P P CSG P CSG ...) by one union or SmoothUnion operator
+        if (R.n_nodes >= 3 && R.n_nodes % 2 == 1) {
+            const smcrt_node* N = d->nodes + R.first_node;
+            const int kind = N[2].kind;
+            bool flat = (kind == SMCRT_UNION || kind == SMCRT_SMOOTH_UNION) && N[0].op == SMCRT_OP_PRIM;
+            for (int i2 = 1; flat && i2 < R.n_nodes; ++i2) {
+                const bool want_csg = (i2 % 2) == 0;
+                flat = want_csg ? (N[i2].op == SMCRT_OP_CSG && N[i2].kind == kind && N[i2].k == N[2].k)
+                                : N[i2].op == SMCRT_OP_PRIM;
+            }
+            if (flat) dr.flat = kind == SMCRT_UNION ? 1 : 2;
         }
         s->regions.push_back(dr);
         s->region_mat.push_back(R.material);
@@ -732,6 +749,42 @@ struct LaunchShape {
     size_t lds;
 };
 
+// The packet kernel is compiled for a few kind sets; a scene runs the smallest one that covers
+// every kind its program uses (a kernel without the unused interpreter cases is about 40% less
+// code, and the kernel is instruction-cache bound).  SMCRT_KERNEL=full in the environment forces
+// the general variant.
+typedef void (*PacketKernelFn)(KParams);
+static constexpr uint32_t KM_BASIC = KM_PRIM(SMCRT_SPHERE) | KM_PRIM(SMCRT_BOX) | KM_CSG(SMCRT_UNION);
+static constexpr uint32_t KM_LAYERS = KM_BASIC | KM_PRIM(SMCRT_CYLINDER) | KM_CSG(SMCRT_SMOOTH_UNION);
+static constexpr uint32_t KM_EXTRUDED = KM_PRIM(SMCRT_BOX) | KM_PRIM(SMCRT_POLYGON) | KM_MOD(SMCRT_EXTRUDE);
+struct KernelVariant {
+    uint32_t mask;
+    const char* name;
+    PacketKernelFn fwd, adj;
+};
+static const KernelVariant kVariants[] = {
+    {KM_BASIC, "basic", packet_kernel<false, KM_BASIC>, packet_kernel<true, KM_BASIC>},
+    {KM_LAYERS, "layers", packet_kernel<false, KM_LAYERS>, packet_kernel<true, KM_LAYERS>},
+    {KM_EXTRUDED, "extruded", packet_kernel<false, KM_EXTRUDED>, packet_kernel<true, KM_EXTRUDED>},
+    {KM_FULL, "full", packet_kernel<false, KM_FULL>, packet_kernel<true, KM_FULL>},
+};
+
+static const KernelVariant& kernel_variant(const smcrt_scene* s) {
+    const size_t nv = sizeof kVariants / sizeof kVariants[0];
+    const char* force = getenv("SMCRT_KERNEL");
+    if (force && !strcmp(force, "full")) return kVariants[nv - 1];
+    for (size_t i = 0; i < nv; ++i)
+        if ((s->kind_mask & ~kVariants[i].mask) == 0u) return kVariants[i];
+    return kVariants[nv - 1];
+}
+
+static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const LaunchShape& sh, hipStream_t stream,
+                                       KParams& P) {
+    const KernelVariant& v = kernel_variant(s);
+    void* args[] = {&P};
+    return hipLaunchKernel((const void*)(adj ? v.adj : v.fwd), dim3(sh.blocks), dim3(sh.block), args, sh.lds, stream);
+}
+
 // Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
 // material tables of n_sets sets, plus the adjoint's per-lane scores and gradients).
 static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0,
@@ -756,8 +809,8 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once
         int active = 0;
-        hipError_t e = adj ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<true>, L.block, L.lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel<false>, L.block, L.lds);
+        const KernelVariant& v = kernel_variant(s);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, adj ? v.adj : v.fwd, L.block, L.lds);
         if (e == hipSuccess && active > 0) bpc = active;
     }
     L.blocks = s->num_cu * bpc;
@@ -910,8 +963,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     k_fill_f64<<<1, 64, 0, stream>>>(Cc.moments, 24);
     if (adj) k_fill_f64<<<1, 256, 0, stream>>>(Cc.grad, 4 * (size_t)P.nmat + 1);
     HIPCHECK(hipEventRecord(s->ev0, stream));
-    if (adj) hipLaunchKernelGGL(packet_kernel<true>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
-    else hipLaunchKernelGGL(packet_kernel<false>, dim3(sh.blocks), dim3(sh.block), sh.lds, stream, P);
+    HIPCHECK(launch_packet_kernel(s, adj, sh, stream, P));
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(s->ev1, stream));
     HIPCHECK(hipStreamSynchronize(stream));
@@ -1029,9 +1081,9 @@ extern "C" int smcrt_evaluate(smcrt_scene* s, int32_t region, int64_t n, const f
     HIPCHECK(hipMemcpy(s->d_pts.p, pts, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
     float* dd = dist ? s->d_out.as<float>() : nullptr;
     float* dn = normals ? s->d_out.as<float>() + n : nullptr;
-    const DevRegion reg = s->regions[region];
     hipLaunchKernelGGL(k_evaluate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, s->d_nodes.as<DevNode>(),
-                       reg, (long long)n, s->d_pts.as<float>(), dd, dn, s->accel.empty() ? nullptr : s->d_accel.as<float>());
+                       s->d_regions.as<DevRegion>(), (int)region, (long long)n, s->d_pts.as<float>(), dd, dn,
+                       s->accel.empty() ? nullptr : s->d_accel.as<float>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s->stream));
     if (dist) HIPCHECK(hipMemcpy(dist, dd, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
@@ -1264,8 +1316,16 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
     k_fill_f64<<<(unsigned)fill_blocks(ndoubles), 256, 0, b->stream>>>(b->d_grids.as<double>(), ndoubles);
     k_fill_u64<<<(unsigned)fill_blocks(nstats), 256, 0, b->stream>>>(b->d_stats.as<unsigned long long>(), nstats);
     k_fill_u64<<<1, 64, 0, b->stream>>>(b->d_work.as<unsigned long long>(), b->groups.size());
-    for (BatchGroup& g : b->groups)
-        hipLaunchKernelGGL(packet_kernel<false>, dim3(g.sh.blocks), dim3(g.sh.block), g.sh.lds, b->stream, g.P);
+    for (BatchGroup& g : b->groups) {
+        hipError_t le = launch_packet_kernel(g.scene, false, g.sh, b->stream, g.P);
+        if (le != hipSuccess) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(b->stream, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            smcrt_batch_destroy(b);
+            return fail(SMCRT_EDEVICE, "packet kernel launch during capture failed: %s", hipGetErrorString(le));
+        }
+    }
     hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
     if (ce != hipSuccess) {
         smcrt_batch_destroy(b);

@@ -29,7 +29,8 @@ struct alignas(16) DevNode {
 static_assert(sizeof(DevNode) == 128, "DevNode must stay 32 dwords");
 
 struct DevRegion {
-    int first, count, material, pad;
+    int first, count, material;
+    int flat;  // 1: flat fold of primitives by union, 2: by SmoothUnion (lazy gradients), 0: other
 };
 
 // Per-region material constants (one table per property set); `mat` is the region's material
@@ -235,23 +236,39 @@ SMCRT_DEV T polygon_eval(const float* __restrict__ A, int base, const V3<T>& p) 
 }
 
 // ------------------------------------------------------------------------------------------
+// Kind masks.  The packet kernel is compiled for a few sets of primitive / operator / modifier
+// kinds (the host launches the smallest variant covering the scene): the interpreter's code is a
+// large part of the kernel, and a kernel whose loop outgrows the instruction cache runs far
+// slower, so scenes that use two primitives get a kernel that contains two.
+// ------------------------------------------------------------------------------------------
+#define KM_PRIM(k) (1u << (k))         // primitive kinds 1..11
+#define KM_CSG(k) (1u << (15 + (k)))   // CSG kinds 1..4 -> bits 16..19
+#define KM_MOD(k) (1u << (23 + (k)))   // modifier kinds 1..6 -> bits 24..29
+#define KM_ANY_CSG (KM_CSG(1) | KM_CSG(2) | KM_CSG(3) | KM_CSG(4))
+#define KM_ANY_MOD (KM_MOD(1) | KM_MOD(2) | KM_MOD(3) | KM_MOD(4) | KM_MOD(5) | KM_MOD(6))
+#define KM_FULL (0xFFEu | KM_ANY_CSG | KM_ANY_MOD)
+#define KM_HAS(KM, bit) (((KM) & (bit)) != 0u)
+
+// ------------------------------------------------------------------------------------------
 // SDF primitives (src/sdfs/sdfs.f90:494-735)
 // ------------------------------------------------------------------------------------------
-template <class T>
+template <class T, uint32_t KM = KM_FULL>
 SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, const float* __restrict__ accel) {
+    // a kind outside the mask is never dispatched (the host picks a covering variant): its case
+    // compiles to a break
     switch (kind) {
-    case 1: {  // sphere
+    case 1: if constexpr (!KM_HAS(KM, KM_PRIM(1))) break; {  // sphere
         return g_sqrt(vdot(p, p)) - P[0];
     }
-    case 2: {  // box
+    case 2: if constexpr (!KM_HAS(KM, KM_PRIM(2))) break; {  // box
         V3<T> q = vsubc(vabs(p), P[0], P[1], P[2]);
         return vlen(vmax0(q)) + g_min0(g_max(q.x, g_max(q.y, q.z)));
     }
-    case 3: {  // torus
+    case 3: if constexpr (!KM_HAS(KM, KM_PRIM(3))) break; {  // torus
         T a = g_sqrt(p.x * p.x + p.z * p.z) - P[0];
         return g_sqrt(a * a + p.y * p.y) - P[1];
     }
-    case 4: {  // cylinder(a, b, r)
+    case 4: if constexpr (!KM_HAS(KM, KM_PRIM(4))) break; {  // cylinder(a, b, r)
         float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
         float baba = bax * bax + bay * bay + baz * baz;
         V3<T> pa = vsubc(p, P[0], P[1], P[2]);
@@ -271,12 +288,12 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
         T r = g_sqrt(g_abs(d)) * (1.0f / baba);
         return g_neg_if(r, val(d) < 0.f);
     }
-    case 5: {  // triprism(h1, h2)
+    case 5: if constexpr (!KM_HAS(KM, KM_PRIM(5))) break; {  // triprism(h1, h2)
         V3<T> q = vabs(p);
         return g_max(q.z - P[1], g_max(q.x * 0.866025f + p.y * 0.5f, -p.y) - P[0] * 0.5f);
     }
     case 6:    // segment (tube thickness 0.1)
-    case 7: {  // capsule(a, b, r)
+    case 7: if constexpr (!KM_HAS(KM, KM_PRIM(6) | KM_PRIM(7))) break; {  // capsule(a, b, r)
         float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
         float baba = bax * bax + bay * bay + baz * baz;
         V3<T> pa = vsubc(p, P[0], P[1], P[2]);
@@ -284,7 +301,7 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
         V3<T> w{pa.x - h * bax, pa.y - h * bay, pa.z - h * baz};
         return vlen(w) - (kind == 6 ? 0.1f : P[6]);
     }
-    case 8: {  // capped cone(a, b, ra, rb)
+    case 8: if constexpr (!KM_HAS(KM, KM_PRIM(8))) break; {  // capped cone(a, b, ra, rb)
         float ra = P[6], rb = P[7], rba = rb - ra;
         float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
         float baba = bax * bax + bay * bay + baz * baz;
@@ -302,7 +319,7 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
         T r = g_sqrt(g_min(cax * cax + baba * (cay * cay), cbx * cbx + baba * (cby * cby)));
         return g_neg_if(r, neg);
     }
-    case 9: {  // egg(r1, r2, h)
+    case 9: if constexpr (!KM_HAS(KM, KM_PRIM(9))) break; {  // egg(r1, r2, h)
         float r1 = P[0], r2 = P[1], h = P[2];
         float r = r1 - r2, hin = h + r;
         float l = (hin * hin - r * r) / (2.f * r);
@@ -315,46 +332,52 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
         T dx = px + l;
         return g_sqrt(dx * dx + p.y * p.y + p.z * p.z) - (r1 + l);
     }
-    case 10: {  // plane(a)
+    case 10: if constexpr (!KM_HAS(KM, KM_PRIM(10))) break; {  // plane(a)
         return p.x * P[0] + p.y * P[1] + p.z * P[2];
     }
     case 11:  // polygon (P[0]: accelerator offset)
-        return polygon_eval<T>(accel, (int)P[0], p);
+        if constexpr (KM_HAS(KM, KM_PRIM(11))) return polygon_eval<T>(accel, (int)P[0], p);
+        break;
     }
     return g_const<T>(0.f);
 }
 
 // CSG operators (src/sdfs/sdfModifiers.f90:412-475)
-template <class T>
+template <class T, uint32_t KM = KM_FULL>
 SMCRT_DEV T csg_eval(int kind, const T& d1, const T& d2, float k) {
     switch (kind) {
-    case 1: return g_min(d1, d2);
-    case 2: {
+    case 1: if constexpr (KM_HAS(KM, KM_CSG(1))) return g_min(d1, d2);
+        break;
+    case 2: if constexpr (KM_HAS(KM, KM_CSG(2))) {
         T h = g_max0(k - g_abs(d1 - d2)) * (1.0f / k);
         return g_min(d1, d2) - h * h * h * (k * (1.0f / 6.0f));
     }
-    case 3: return g_max(-d1, d2);
-    case 4: return g_max(d1, d2);
+        break;
+    case 3: if constexpr (KM_HAS(KM, KM_CSG(3))) return g_max(-d1, d2);
+        break;
+    case 4: if constexpr (KM_HAS(KM, KM_CSG(4))) return g_max(d1, d2);
+        break;
     }
     return d1;
 }
 
 // modifier position maps (src/sdfs/sdfModifiers.f90:290-375)
-template <class T>
+template <class T, uint32_t KM = KM_FULL>
 SMCRT_DEV V3<T> mod_pos(int kind, const float* __restrict__ P, const V3<T>& p) {
     switch (kind) {
-    case 2: {  // revolution
+    case 2: if constexpr (!KM_HAS(KM, KM_MOD(2))) break; {  // revolution
         T a = g_sqrt(p.x * p.x + p.z * p.z) - P[0];
         return V3<T>{a, p.y, g_const<T>(0.f)};
     }
     case 4:  // elongate
-        return vmax0(vsubc(vabs(p), P[0], P[1], P[2]));
-    case 5: {  // twist
+        if constexpr (KM_HAS(KM, KM_MOD(4))) return vmax0(vsubc(vabs(p), P[0], P[1], P[2]));
+        break;
+    case 5: if constexpr (!KM_HAS(KM, KM_MOD(5))) break; {  // twist
         T a = p.z * P[0];
         T c = g_cos(a), s = g_sin(a);
         return V3<T>{c * p.x - s * p.y, s * p.x + c * p.y, p.z};
     }
-    case 6: {  // bend
+    case 6: if constexpr (!KM_HAS(KM, KM_MOD(6))) break; {  // bend
         T a = p.x * P[0];
         T c = g_cos(a), s = g_sin(a);
         return V3<T>{c * p.x - s * p.y, s * p.x + c * p.y, p.z};
@@ -364,17 +387,18 @@ SMCRT_DEV V3<T> mod_pos(int kind, const float* __restrict__ P, const V3<T>& p) {
 }
 
 // modifier post-processing (src/sdfs/sdfModifiers.f90:273-335)
-template <class T>
+template <class T, uint32_t KM = KM_FULL>
 SMCRT_DEV T mod_post(int kind, const float* __restrict__ P, const V3<T>& p, const T& d) {
     switch (kind) {
-    case 1: {  // extrude
+    case 1: if constexpr (!KM_HAS(KM, KM_MOD(1))) break; {  // extrude
         T wy = g_abs(p.z) - P[0];
         T mx = g_max0(d), my = g_max0(wy);
         return g_min0(g_max(d, wy)) + g_sqrt(mx * mx + my * my);
     }
     case 3:  // onion
-        return g_abs(d) - P[0];
-    case 4: {  // elongate
+        if constexpr (KM_HAS(KM, KM_MOD(3))) return g_abs(d) - P[0];
+        break;
+    case 4: if constexpr (!KM_HAS(KM, KM_MOD(4))) break; {  // elongate
         V3<T> q = vsubc(vabs(p), P[0], P[1], P[2]);
         return d + g_min0(g_max(q.x, g_max(q.y, q.z)));
     }
@@ -392,7 +416,7 @@ SMCRT_DEV V3<T> xform(const float* __restrict__ t, const V3<T>& p) {
 // Evaluate one region's postfix program at p.  Node range and node contents are wave-uniform,
 // so the opcode dispatch is a scalar branch and parameters come from scalar loads.  The value
 // and position stacks are shift registers (no dynamically indexed register arrays).
-template <class T>
+template <class T, uint32_t KM = KM_FULL>
 SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, const float* __restrict__ accel) {
     T v0 = g_const<T>(0.f), v1 = v0, v2 = v0, v3 = v0;
     V3<T> q0 = p0, q1 = p0, q2 = p0;  // q0 = top of the position stack
@@ -410,17 +434,23 @@ SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, 
                 for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
                 q = xform<T>(tt, q0);
             }
-            T d = prim_eval<T>(kind, pp, q, accel);
+            T d = prim_eval<T, KM>(kind, pp, q, accel);
             v3 = v2; v2 = v1; v1 = v0; v0 = d;
         } else if (op == 2) {  // CSG: d1 = v1, d2 = v0
-            T d = csg_eval<T>(kind, v1, v0, n->k);
-            v0 = d; v1 = v2; v2 = v3;
+            if constexpr (KM_HAS(KM, KM_ANY_CSG)) {
+                T d = csg_eval<T, KM>(kind, v1, v0, n->k);
+                v0 = d; v1 = v2; v2 = v3;
+            }
         } else if (op == 3) {  // MOD_BEGIN
-            V3<T> q = mod_pos<T>(kind, pp, q0);
-            q2 = q1; q1 = q0; q0 = q;
+            if constexpr (KM_HAS(KM, KM_ANY_MOD)) {
+                V3<T> q = mod_pos<T, KM>(kind, pp, q0);
+                q2 = q1; q1 = q0; q0 = q;
+            }
         } else {  // MOD_END
-            q0 = q1; q1 = q2;
-            v0 = mod_post<T>(kind, pp, q0, v0);
+            if constexpr (KM_HAS(KM, KM_ANY_MOD)) {
+                q0 = q1; q1 = q2;
+                v0 = mod_post<T, KM>(kind, pp, q0, v0);
+            }
         }
     }
     return v0;

@@ -294,3 +294,48 @@ def test_run_device_matches_run(gpu):
     # the LDS tile accumulates in FP32 in a scheduling-dependent order: equal to ~1e-7
     np.testing.assert_allclose(d.cpu().numpy().reshape(g.shape, order="F"), r.jmean, rtol=1e-6, atol=1e-12)
     assert abs(st["tally_sum"] / r.jmean.sum() - 1) < 1e-6
+
+
+def test_fold_normals_match_oracle(gpu):
+    """Normals of flat union / SmoothUnion folds of many primitives (the lazy forward-mode path
+    through the fold) against the oracle: overlapping spheres with a wide smoothing radius (most
+    points in blend zones of two or three children) and the eight vessels of the skin scene."""
+    centres = [(-0.5, 0.0, 0.0), (-0.2, 0.2, 0.1), (0.1, -0.1, 0.0), (0.4, 0.1, -0.1), (0.0, 0.3, 0.3)]
+    blobs_s = model([sphere(0.3, M, 1, transform=invert(translate(c))) for c in centres], SmoothUnion, 0.3)
+    blobs_u = model([sphere(0.3, M, 1, transform=invert(translate(c))) for c in centres], union)
+    skin, _ = scenes.build("skin_vessels", nxyz=16)
+    sc = Scene([blobs_s, blobs_u, skin.array[0]], cart_grid(10, 10, 10, 2, 2, 2), source("point"))
+    eng = Engine(sc)
+    rng = np.random.default_rng(5)
+    pts = {0: rng.uniform(-0.9, 0.9, (4000, 3)), 1: rng.uniform(-0.9, 0.9, (4000, 3))}
+    # points around the vessels (cylinders along x at the drawn y and depths)
+    v = rng.uniform(-1, 1, (4000, 3)) * [0.9, 0.9, 0.5]
+    pts[2] = v
+    for r, p in pts.items():
+        _, ng = eng.evaluate(r, p, normals=True)
+        _, no = O.evaluate(sc, r, p, normals=True)
+        err = np.linalg.norm(ng - no, axis=1)
+        assert np.mean(err > 1e-4) < 0.01, (r, np.mean(err > 1e-4))
+        assert np.median(err) < 1e-5, (r, np.median(err))
+
+
+@pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels", "polygon_regions"])
+def test_kernel_variants_agree(gpu, name, monkeypatch):
+    """The packet kernel specialised to the scene's kinds (sphere/box, layered vessels, extruded
+    polygons) and the general kernel (SMCRT_KERNEL=full) run the same packets to the same
+    termination counts, the same fluence (to the FP32 tile order) and the same adjoint sums."""
+    kw = {"nxyz": 32} if name == "skin_vessels" else ({"n_side": 2, "n_edges": 40, "nxy": 40, "nz": 8}
+                                                      if name == "polygon_regions" else {})
+    sc, _ = scenes.build(name, **kw)
+    eng = Engine(sc)
+    a = eng.run(200_000, seed=23)
+    ga = eng.grad(50_000, seed=29).d_w_dmua
+    monkeypatch.setenv("SMCRT_KERNEL", "full")
+    b = eng.run(200_000, seed=23)
+    gb = eng.grad(50_000, seed=29).d_w_dmua
+    for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface", "scatters", "ev_steps", "ev_crossings"):
+        assert a.stats[k] == b.stats[k], k
+    assert abs(a.jmean.sum() / b.jmean.sum() - 1) < 1e-6
+    m = b.jmean > 1e-3 * b.jmean.max()
+    assert np.max(np.abs(a.jmean[m] / b.jmean[m] - 1)) < 1e-4
+    np.testing.assert_allclose(ga, gb, rtol=1e-6, atol=1e-12 * np.abs(gb).max())

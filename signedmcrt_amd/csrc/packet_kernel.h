@@ -116,6 +116,7 @@ struct Lane {
     float opx, opy, opz;  // position before the crossing step
     float tau, taurun, dlay_old, dprev;
     float weight;         // packet weight (the weight_scatter estimator; 1 otherwise)
+    float rx, ry, rz;     // rounding error of the march moves so far (see move_axis)
     int layer, st, bounces, nsc, set, tcount;
     bool iface;           // the current flight started at an interface
     bool ghost;           // left the grid mid-flight: moves on, untallied, to the flight's end
@@ -132,6 +133,26 @@ struct Walk {
     bool active, flight_over, inside;
     float aint;  // adjoint: integral of C over the step
 };
+
+// p += inc with the rounding error carried in r (Knuth's TwoSum).  A march step is the distance
+// to the nearest surface; a packet leaving a surface at a grazing angle takes steps of its height
+// h above it and rises h*|cos| per step.  Below half an ulp of the coordinate that rise rounds
+// away in FP32, the height never grows and the packet crawls along the surface in steps of h
+// (1e5 steps for 0.1 cm at h = 1e-6).  Carried, the rises accumulate and the height grows
+// geometrically, as in the reference's FP64 arithmetic.
+SMCRT_DEV void move_axis(float& p, float& r, float inc) {
+    const float b = inc + r;
+    const float s = p + b;
+    const float bb = s - p;
+    r = (p - (s - bb)) + (b - bb);
+    p = s;
+}
+
+SMCRT_DEV void move(Lane& L, float d) {
+    move_axis(L.px, L.rx, d * L.dx);
+    move_axis(L.py, L.ry, d * L.dy);
+    move_axis(L.pz, L.rz, d * L.dz);
+}
 
 SMCRT_DEV float rnd(Lane& L, const KParams& P) { return rng_u(L.rng, P.seed_lo, P.seed_hi); }
 
@@ -451,7 +472,9 @@ SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
 }
 
 template <bool ADJ, uint32_t KM>
-__global__ void __launch_bounds__(256) packet_kernel(KParams P) {
+// at least 2 resident waves per SIMD (at most 256 VGPRs; the general variant would otherwise
+// take more registers for 1)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     CDevNode* const nodes = (CDevNode*)P.nodes;
     CKCold* const C = (CKCold*)P.cold;
@@ -567,6 +590,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
         if (L.st == ST_EMIT) {
             rng_init(L.rng, L.pid);
             emit(P, C, L);
+            L.rx = 0.f; L.ry = 0.f; L.rz = 0.f;
             L.bounces = 0;
             L.nsc = 0;
             L.tcount = 0;
@@ -753,7 +777,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
             if (W.flight_over) d = (L.tau - L.taurun) / kap;
             else L.taurun += t;
             if (L.ghost) {  // outside the grid: move without a walk (nothing to tally)
-                L.px += d * L.dx; L.py += d * L.dy; L.pz += d * L.dz;
+                move(L, d);
                 if (W.flight_over) end_kind = classify_escape(C, L);
                 else L.st = ST_MARCHED;
             } else {
@@ -832,7 +856,7 @@ __global__ void __launch_bounds__(256) packet_kernel(KParams P) {
                     adj_sc(A, 1, m, P.nmat) += W.aint;
                 }
             }
-            L.px += d * L.dx; L.py += d * L.dy; L.pz += d * L.dz;
+            move(L, d);
             if (!W.inside) {
                 // left the grid (src/inttau2.f90:256-259,282-285): the outcome is fixed.  With
                 // detectors the packet moves on, untallied, to the end of its flight or the next

@@ -881,8 +881,18 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
     int rc = launch_run(s, &po, pilot, false, nullptr, stream, &ms);
     std::vector<double> h(nv);
     if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
-        // 64 KiB of FP32 LDS per block leaves room for two 256-lane blocks per CU
-        choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, 16384, s->tile_t0, s->tile_tn);
+        // the tile takes the LDS left per 256-lane block at the kernel's register occupancy
+        // (2 or 3 blocks per CU), at most 64 KiB of FP32
+        int lds_cu = 0, bpc = 0;
+        if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
+            lds_cu <= 0)
+            lds_cu = 160 * 1024;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_variant(s).fwd, 256, 0) != hipSuccess || bpc < 1)
+            bpc = 2;
+        const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
+        const long per_block = (long)lds_cu / bpc - mats - 256;
+        const int budget = (int)std::max(0L, std::min(16384L, per_block / (long)sizeof(float)));
+        if (budget > 0) choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0, s->tile_tn);
     }
     (void)hipFree(pilot);
     return rc;

@@ -258,7 +258,7 @@ def main():
                    "launch_shape": tuned["shape"] if tuned else "library default"},
         "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP32_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": "packet_kernel<false>",
+                     "traffic": traffic, "kernel": "packet_kernel<false, %du> (%s)" % (eng.kernel_variant()[2], eng.kernel_variant()[0]),
                      "kernel_ms": kern_ms, "flops_per_launch": flops_launch},
         "roofline_tally": {"bound": "hbm", "achieved": tally_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": tally_gbs / HBM_PEAK_GBS,

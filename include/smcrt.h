@@ -345,6 +345,13 @@ int smcrt_render(smcrt_scene* scene, const int32_t samples[3], float* image);
 /* Number of bins of detector `i` of a scene (0 if out of range). */
 int64_t smcrt_detector_bins(const smcrt_scene* scene, int32_t i);
 
+/* Diagnostics (no reference counterpart): the packet-kernel variant a scene runs.  The kernel
+   is compiled for a few sets of primitive/operator/modifier kinds; a scene runs the smallest
+   one covering the kinds its program uses.  Writes the scene's kind mask (bit k: primitive k;
+   bit 15+k: operator k; bit 23+k: modifier k) and the variant's mask, and returns its name
+   ("basic", "layers", "extruded" or "full"; NULL for a NULL scene). */
+const char* smcrt_scene_kernel(const smcrt_scene* scene, uint32_t* scene_mask, uint32_t* variant_mask);
+
 /* ---- input: the reference's TOML schema (src/parse/parse.f90, parse_source.f90,
  * parse_detectors.f90) -------------------------------------------------------------------
  * smcrt_config_load reads a TOML input file into a flat, caller-owned struct, applying the

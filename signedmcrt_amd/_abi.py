@@ -162,7 +162,8 @@ EXPORTS = ["smcrt_abi_version", "smcrt_last_error", "smcrt_sizeof", "smcrt_scene
            "smcrt_scene_set_materials", "smcrt_run", "smcrt_run_device", "smcrt_evaluate",
            "smcrt_grad", "smcrt_batch_create", "smcrt_batch_run", "smcrt_batch_read",
            "smcrt_batch_destroy", "smcrt_normalise_fluence", "smcrt_write_nrrd",
-           "smcrt_detector_bins", "smcrt_config_load", "smcrt_write_detector", "smcrt_render"]
+           "smcrt_detector_bins", "smcrt_config_load", "smcrt_write_detector", "smcrt_render",
+           "smcrt_scene_kernel"]
 
 _lib = None
 
@@ -207,6 +208,7 @@ def lib() -> C.CDLL:
         "smcrt_sizeof": (C.c_int64, [C.c_char_p]),
         "smcrt_detector_bins": (C.c_int64, [C.c_void_p, C.c_int32]),
         "smcrt_render": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_float)]),
+        "smcrt_scene_kernel": (C.c_char_p, [C.c_void_p, P(C.c_uint32), P(C.c_uint32)]),
         "smcrt_config_load": (C.c_int, [C.c_char_p, P(Config)]),
         "smcrt_write_detector": (C.c_int, [C.c_char_p, P(Detector), P(C.c_uint64)]),
     }

@@ -53,9 +53,12 @@ def run_kwargs(shape: Dict) -> Dict:
 
 
 def coordinate_descent(cost: Callable[[Dict], float], space: Dict[str, Sequence],
-                       start: Dict, sweeps: int = 2) -> Tuple[Dict, List[Tuple[Dict, float]]]:
+                       start: Dict, sweeps: int = 2, min_gain: float = 0.0
+                       ) -> Tuple[Dict, List[Tuple[Dict, float]]]:
     """Minimise `cost` over the product space one knob at a time, `sweeps` passes over the
-    knobs; each point is measured once (memoised).  Returns (best shape, [(shape, cost)])."""
+    knobs; each point is measured once (memoised).  A candidate replaces the best point only if
+    it is cheaper by more than the fraction `min_gain` (timing noise does not move the shape).
+    Returns (best shape, [(shape, cost)])."""
     memo: Dict[Tuple, float] = {}
     table: List[Tuple[Dict, float]] = []
 
@@ -77,7 +80,7 @@ def coordinate_descent(cost: Callable[[Dict], float], space: Dict[str, Sequence]
                 cand = dict(best)
                 cand[knob] = v
                 c = f(cand)
-                if c < best_c:
+                if c < best_c * (1.0 - min_gain):
                     best, best_c, changed = cand, c, True
         if not changed:
             break
@@ -127,11 +130,16 @@ def _device_name() -> str:
 
 def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
          cache_path: Optional[str] = None, use_cache: bool = True,
-         space: Optional[Dict[str, Sequence]] = None, seed: int = 7919) -> TuneResult:
+         space: Optional[Dict[str, Sequence]] = None, seed: int = 7919,
+         target_ms: float = 150.0, min_gain: float = 0.02) -> TuneResult:
     """Tune the launch shape of `engine`'s scene on the current GPU.
 
-    n_probe: packets per probe (default: enough for ~30 ms of kernel time at the default shape,
-    estimated from a first probe).  The probe tallies into a scratch device grid."""
+    n_probe: packets per probe (default: enough for `target_ms` of kernel time at the default
+    shape, estimated from a first probe).  A persistent launch fills and drains: for the last
+    few milliseconds of every launch most lanes are idle, which favours small blocks and chunks
+    in short probes but not in long runs, so the probes are long enough for the steady state to
+    dominate.  A knob moves away from the library default only for a gain above `min_gain`.
+    The probe tallies into a scratch device grid."""
     import torch
 
     space = dict(space or SPACE)
@@ -167,9 +175,13 @@ def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
         n0 = 200_000
         probe(start, n0)  # warm-up: the LDS tile's pilot run, code load
         ms = probe(start, n0)
-        n_probe = int(min(max(n0 * 30.0 / max(ms, 1e-3), 200_000), 20_000_000))
+        # a second calibration closer to the target size (short launches are fill/drain bound
+        # and would underestimate the rate)
+        n1 = int(min(max(n0 * 0.25 * target_ms / max(ms, 1e-3), n0), 50_000_000))
+        ms = probe(start, n1)
+        n_probe = int(min(max(n1 * target_ms / max(ms, 1e-3), 200_000), 50_000_000))
     t0 = time.perf_counter()
-    best, table = coordinate_descent(lambda sh: probe(sh, n_probe), space, start, sweeps)
+    best, table = coordinate_descent(lambda sh: probe(sh, n_probe), space, start, sweeps, min_gain)
     best_ms = dict((tuple(sorted(s.items())), c) for s, c in table)[tuple(sorted(best.items()))]
     default_ms = table[0][1]
     res = TuneResult(best, best_ms, default_ms, table)

@@ -1430,3 +1430,11 @@ extern "C" int smcrt_write_nrrd(const char* path, const float* data, int32_t nx,
     if (bad) return fail(SMCRT_EIO, "short write to %s", path);
     return SMCRT_OK;
 }
+
+extern "C" const char* smcrt_scene_kernel(const smcrt_scene* s, uint32_t* scene_mask, uint32_t* variant_mask) {
+    if (!s) return nullptr;
+    const KernelVariant& v = kernel_variant(s);
+    if (scene_mask) *scene_mask = s->kind_mask;
+    if (variant_mask) *variant_mask = v.mask;
+    return v.name;
+}

@@ -151,6 +151,12 @@ class Engine:
                                            C.byref(st), C.c_void_p(stream)))
         return st.as_dict()
 
+    def kernel_variant(self):
+        """(name, scene kind mask, variant kind mask) of the packet kernel this scene runs."""
+        sm, vm = C.c_uint32(), C.c_uint32()
+        name = self._lib.smcrt_scene_kernel(self.handle, C.byref(sm), C.byref(vm))
+        return name.decode(), sm.value, vm.value
+
     # -- SDF evaluation ----------------------------------------------------------------------
     def evaluate(self, region: int, pts, normals: bool = False):
         p = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)

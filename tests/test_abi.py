@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "smcrt.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t)\s+(smcrt_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\s*\*)\s*(smcrt_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_and_mirror_agree():
@@ -112,3 +112,41 @@ def test_nrrd_layout(tmp_path):
                            "encoding: raw\nendian: little\nnphotons = 10\n")
     assert head.endswith("\n\n\n")
     np.testing.assert_array_equal(np.frombuffer(raw[-96:], dtype="<f4"), data.reshape(-1, order="F"))
+
+
+@pytest.mark.parametrize("name, kw, want", [
+    ("refracting_sphere", {}, "basic"), ("slab", {}, "basic"), ("scat_test", {}, "basic"),
+    ("skin_vessels", {"nxyz": 16}, "layers"),
+    ("polygon_regions", {"n_side": 2, "n_edges": 12, "nxy": 16, "nz": 4}, "extruded"),
+])
+def test_scene_kernel_variant(name, kw, want, monkeypatch):
+    """Each scene runs the smallest packet-kernel variant covering its kinds; SMCRT_KERNEL=full
+    forces the general one (no GPU needed: the choice is made on the host)."""
+    sc, _ = scenes.build(name, **kw)
+    lib = A.lib()
+    h = C.c_void_p()
+    assert lib.smcrt_scene_create(C.byref(sc.desc), C.byref(h)) == A.OK
+    sm, vm = C.c_uint32(), C.c_uint32()
+    try:
+        assert lib.smcrt_scene_kernel(h, C.byref(sm), C.byref(vm)).decode() == want
+        assert sm.value & ~vm.value == 0 and sm.value != 0
+        monkeypatch.setenv("SMCRT_KERNEL", "full")
+        assert lib.smcrt_scene_kernel(h, C.byref(sm), C.byref(vm)).decode() == "full"
+    finally:
+        lib.smcrt_scene_destroy(h)
+    assert lib.smcrt_scene_kernel(None, None, None) is None
+
+
+def test_general_kinds_run_the_full_variant():
+    from signedmcrt_amd.sdf import torus
+    sc = Scene([torus(0.5, 0.2, mono(1, 0.1, 0, 1), 1), box((2, 2, 2), mono(0, 0, 0, 1), 2)],
+               cart_grid(4, 4, 4, 1, 1, 1), source("point"))
+    lib = A.lib()
+    h = C.c_void_p()
+    assert lib.smcrt_scene_create(C.byref(sc.desc), C.byref(h)) == A.OK
+    sm = C.c_uint32()
+    try:
+        assert lib.smcrt_scene_kernel(h, C.byref(sm), None).decode() == "full"
+        assert sm.value == (1 << 3) | (1 << 2)  # torus, box
+    finally:
+        lib.smcrt_scene_destroy(h)

@@ -187,12 +187,16 @@ enum {
     SMCRT_FLAG_END_EARLY = 1u << 1, /* terminate a packet at its 5th scatter (same harness)   */
     SMCRT_FLAG_NO_TALLY = 1u << 2,  /* skip the fluence tally                                  */
     SMCRT_FLAG_NO_LDS_TILE = 1u << 3, /* tally straight to HBM (no LDS-privatised tile)         */
-    SMCRT_FLAG_WEIGHT = 1u << 4     /* the `weight_scatter` estimator (src/kernelsMod.f90:12-163):
+    SMCRT_FLAG_WEIGHT = 1u << 4,    /* the `weight_scatter` estimator (src/kernelsMod.f90:12-163):
                                        at each interaction deposit w(1 - albedo) into its voxel
                                        (on top of the path-length tally, as the reference does),
                                        always scatter, Russian roulette below w = 0.01 (survival
                                        0.1, weight x10); Fresnel events scale w by R; detectors
                                        bin the final weight.  Roulette kills count as absorbed. */
+    SMCRT_FLAG_SHARED_GRID = 1u << 5 /* tally every block into the one grid (default: one FP64
+                                       partial grid per XCD, summed after the launch; same
+                                       result up to FP64 summation order, ~2x faster where
+                                       tally atomics dominate; costs n_xcd x grid memory)      */
 };
 
 typedef struct smcrt_run_opts {

@@ -29,6 +29,7 @@ FLAG_END_EARLY = 1 << 1
 FLAG_NO_TALLY = 1 << 2
 FLAG_NO_LDS_TILE = 1 << 3
 FLAG_WEIGHT = 1 << 4
+FLAG_SHARED_GRID = 1 << 5
 
 MAX_REGIONS = 64
 MAX_NODES = 4096

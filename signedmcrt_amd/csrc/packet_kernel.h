@@ -96,6 +96,12 @@ struct KParams {
     float sx, sy, sz;  // voxels per cm
     double* jmean;     // set 0 grid (nullptr: no tally)
     unsigned long long grid_stride;
+    // per-XCD partial grids: block b tallies into xgrid + (b % xcopies) * xstride (same layout as
+    // jmean), summed into jmean after the launch; nullptr: straight into jmean.  Atomics from
+    // different XCDs on one cache line are far slower than atomics within one XCD.
+    double* xgrid;
+    unsigned long long xstride;
+    int xcopies;
     float eps;       // march / crossing epsilon
     float eps_kill;  // flight-start kill threshold after emission or scatter
     // run
@@ -262,17 +268,22 @@ SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, 
     uint64_t pending = wave_ballot(active);
     while (pending) {
         int leader = __builtin_ctzll(pending);
-        int r = __builtin_amdgcn_readlane(r_lane, leader);
-        bool mine = active && r_lane == r;
-        if (mine) res = region_eval<T, KM>(nodes, regions[r].first, regions[r].count, p, accel);
+        const int r = __builtin_amdgcn_readlane(r_lane, leader);
+        const bool mine = active && r_lane == r;
+        // the region's bounds are wave-uniform: read them (scalar) before the divergent branch
+        const int first = __builtin_amdgcn_readfirstlane(regions[r].first);
+        const int count = __builtin_amdgcn_readfirstlane(regions[r].count);
+        if (mine) res = region_eval<T, KM>(nodes, first, count, p, accel);
         pending &= ~wave_ballot(mine);
     }
     return res;
 }
 
-// One primitive node at p: transform (unless the identity), then the primitive.
+// One primitive node (index i, wave-uniform) at p: transform (unless the identity), then the
+// primitive.  readfirstlane keeps the node reads scalar wherever the call sits.
 template <class T, uint32_t KM>
-SMCRT_DEV T prim_node(CDevNode* n, const V3<T>& p, const float* __restrict__ accel) {
+SMCRT_DEV T prim_node(CDevNode* nodes, int i, const V3<T>& p, const float* __restrict__ accel) {
+    CDevNode* n = nodes + __builtin_amdgcn_readfirstlane(i);
     float pp[8], tt[12];
 #pragma unroll
     for (int j = 0; j < 8; ++j) pp[j] = n->p[j];
@@ -294,21 +305,24 @@ SMCRT_DEV V3<float> dual_grad(const Dual& d) { return V3<float>{d.x, d.y, d.z}; 
 // primitive) is computed only where beta != 0.  The walk starts at the last child that wins
 // outright (alpha = 0 there), found in a first float pass, so children that cannot shape the
 // surface at p cost one float evaluation.  Any other program runs the Dual interpreter once.
+//
+// The program's node indices are wave-uniform; readfirstlane keeps them (and so every node read)
+// scalar inside the waterfall's divergent branches.
 template <uint32_t KM>
-SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, CDevRegion* R, const V3<float>& p, const float* __restrict__ accel) {
-    const int first = R->first, count = R->count, flat = R->flat;
+SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int flat, const V3<float>& p,
+                                    const float* __restrict__ accel) {
     const V3<Dual> pd{Dual{p.x, 1.f, 0.f, 0.f}, Dual{p.y, 0.f, 1.f, 0.f}, Dual{p.z, 0.f, 0.f, 1.f}};
     // flat folds exist only in programs with a union or SmoothUnion
     if constexpr (!KM_HAS(KM, KM_CSG(1) | KM_CSG(2))) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
     if (!flat) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
     const int nch = (count + 1) / 2;
     const bool smooth = KM_HAS(KM, KM_CSG(2)) && flat == 2;
-    const float k = smooth ? nodes[first + 2].k : 0.f;
+    const float k = smooth ? nodes[__builtin_amdgcn_readfirstlane(first + 2)].k : 0.f;
     // pass 1: index of the last child that wins outright
-    float a = prim_node<float, KM>(nodes + first, p, accel);
+    float a = prim_node<float, KM>(nodes, first, p, accel);
     int last = 0;
     for (int j = 1; j < nch; ++j) {
-        const float c = prim_node<float, KM>(nodes + first + 2 * j - 1, p, accel);
+        const float c = prim_node<float, KM>(nodes, first + 2 * j - 1, p, accel);
         const bool wins = smooth ? (a - c >= k) : (c < a);
         last = wins ? j : last;
         a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k) : fminf(a, c);
@@ -325,7 +339,7 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, CDevRegion* R, const V3<flo
             const int ls = __builtin_amdgcn_readlane(li, lead);
             const bool mine = li == ls;
             if (mine) {
-                const Dual d = prim_node<Dual, KM>(nodes + ls, pd, accel);
+                const Dual d = prim_node<Dual, KM>(nodes, ls, pd, accel);
                 a = d.v;
                 g = dual_grad(d);
             }
@@ -334,8 +348,8 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, CDevRegion* R, const V3<flo
     }
     for (int j = 1; j < nch; ++j) {
         if (j <= last) continue;
-        CDevNode* n = nodes + first + 2 * j - 1;
-        const float c = prim_node<float, KM>(n, p, accel);
+        const int ni = first + 2 * j - 1;
+        const float c = prim_node<float, KM>(nodes, ni, p, accel);
         float alpha, beta;
         if (smooth) {
             const float h = fmaxf(k - fabsf(a - c), 0.f) * (1.0f / k), h2 = 0.5f * h * h;
@@ -346,7 +360,7 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, CDevRegion* R, const V3<flo
             beta = c < a ? 1.f : 0.f;
         }
         V3<float> gc{0.f, 0.f, 0.f};
-        if (beta != 0.f) gc = dual_grad(prim_node<Dual, KM>(n, pd, accel));
+        if (beta != 0.f) gc = dual_grad(prim_node<Dual, KM>(nodes, ni, pd, accel));
         g = V3<float>{alpha * g.x + beta * gc.x, alpha * g.y + beta * gc.y, alpha * g.z + beta * gc.z};
         a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k) : fminf(a, c);
     }
@@ -363,7 +377,10 @@ SMCRT_DEV V3<float> lane_region_gradient(CDevNode* nodes, CDevRegion* regions, b
         const int leader = __builtin_ctzll(pending);
         const int r = __builtin_amdgcn_readlane(r_lane, leader);
         const bool mine = active && r_lane == r;
-        if (mine) res = region_gradient<KM>(nodes, regions + r, p, accel);
+        const int first = __builtin_amdgcn_readfirstlane(regions[r].first);
+        const int count = __builtin_amdgcn_readfirstlane(regions[r].count);
+        const int flat = __builtin_amdgcn_readfirstlane(regions[r].flat);
+        if (mine) res = region_gradient<KM>(nodes, first, count, flat, p, accel);
         pending &= ~wave_ballot(mine);
     }
     return res;
@@ -452,6 +469,45 @@ SMCRT_DEV int classify_escape(CKCold* C, const Lane& L) {
 
 // Set up the voxel walk of a step of length d from the lane's position.  Returns false when the
 // start lies outside the grid (the packet ends there, src/inttau2.f90:256-259).
+// Global tally atomics, staged per wavefront.  A global atomic instruction costs the wavefront
+// about the same whatever its number of active lanes, and a walk step hands only a few lanes'
+// chords to the grid (the rest merge or land in the LDS tile), so lanes append (voxel, length)
+// pairs to the wavefront's LDS queue and every 64 entries one atomic instruction with all 64
+// lanes active drains them.  push/drain run in wave-converged code only (`n` is wave-uniform);
+// between calls the queue holds < 64 entries, so 128 entries per wavefront suffice.
+constexpr int TALLY_QUEUE = 128;
+struct TallyQueue {
+    uint2* e;  // this wavefront's entries: (grid element index, FP32 length bits)
+    int n;     // entries held (wave-uniform)
+};
+
+SMCRT_DEV void tally_drain(TallyQueue& Q, double* __restrict__ base, int lane, bool all) {
+    while (Q.n >= 64 || (all && Q.n > 0)) {
+        const int n = Q.n < 64 ? Q.n : 64;
+        if (lane < n) {
+            const uint2 e = Q.e[lane];
+            unsafeAtomicAdd(base + e.x, (double)__uint_as_float(e.y));
+        }
+        const int rem = Q.n - n;
+        if (lane < rem) {
+            const uint2 e = Q.e[n + lane];
+            Q.e[lane] = e;
+        }
+        Q.n = rem;
+    }
+}
+
+SMCRT_DEV void tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, double* __restrict__ base, int lane) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return;
+    if (want) {
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        Q.e[Q.n + (int)rank] = make_uint2(idx, __float_as_uint(v));
+    }
+    Q.n += __builtin_popcountll(m);
+    if (Q.n >= 64) tally_drain(Q, base, lane, false);
+}
+
 SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
     const float gx = (L.px + P.xmax) * P.sx, gy = (L.py + P.ymax) * P.sy, gz = (L.pz + P.zmax) * P.sz;
     W.ix = (int)floorf(gx);
@@ -488,6 +544,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
     float* const s_tile = reinterpret_cast<float*>(smem + sizeof(DevMat) * n_mats);
     const int tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) s_tile[i] = 0.f;
+    // this block's tally target (its XCD's partial grid; workgroups go to XCDs round-robin)
+    double* const tbase = P.xgrid ? P.xgrid + (size_t)(blockIdx.x % (unsigned)P.xcopies) * P.xstride : P.jmean;
+    // each wavefront's tally queue comes last (after the tile and the adjoint's scores)
+    TallyQueue Q;
+    Q.n = 0;
+    {
+        size_t qoff = sizeof(DevMat) * n_mats + sizeof(float) * ((tile_vox + 3) & ~3);
+        if (ADJ) qoff += sizeof(float) * 4 * P.nmat * blockDim.x + sizeof(double) * 4 * P.nmat;
+        Q.e = reinterpret_cast<uint2*>(smem + qoff) + (threadIdx.x / 64) * TALLY_QUEUE;
+    }
     AdjLds A;
     if (ADJ) {
         A.stride = blockDim.x;
@@ -524,6 +590,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
     unsigned long long base = 0;
     unsigned avail = 0;
     bool exhausted = false;
+    // one set: packet id = first id + queue position (read once, scalar)
+    const unsigned long long first0 = C->set_first[0] - C->set_prefix[0];
     const float eps = P.eps;
     const float eps_kill = P.eps_kill;
     const unsigned chunk = (unsigned)P.chunk;
@@ -552,6 +620,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                 if (L.st == ST_IDLE && rank < avail) {
                     unsigned long long w = base + rank;
                     int s = 0;
+                    unsigned long long pid = first0 + w;
                     if (P.nsets > 1) {
                         int lo = 0, hi = P.nsets - 1;
                         while (lo < hi) {
@@ -559,6 +628,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                             if (C->set_prefix[mid] <= w) lo = mid; else hi = mid - 1;
                         }
                         s = lo;
+                        pid = C->set_first[s] + (w - C->set_prefix[s]);
                     }
                     if (s != cur_set) {
 #pragma unroll
@@ -569,7 +639,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                         cur_set = s;
                     }
                     L.set = s;
-                    L.pid = C->set_first[s] + (w - C->set_prefix[s]);
+                    L.pid = pid;
                     L.st = ST_EMIT;
                 }
                 unsigned taken = cnt < avail ? cnt : avail;
@@ -787,10 +857,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
         }
 
         // ---- voxel walk: at most `cap` chords per lane this iteration ----
-        bool walk_done = false;
-        if (W.active) {
-            if (W.inside) {
-                for (int k = 0; k < cap; ++k) {
+        // (the loop is wave-uniform so the tally queue can be filled between lane steps)
+        bool walk_done = W.active && !W.inside;  // started outside the grid
+        bool walking = W.active && W.inside;
+        const uint32_t gbase = (uint32_t)L.set * (uint32_t)P.grid_stride;
+        for (int k = 0; k < cap; ++k) {
+            if (wave_ballot(walking) == 0ull) break;
+            bool gpush = false;  // this lane hands its merged chord to the global tally
+            uint32_t gidx = 0u;
+            float glen = 0.f;
+            if (walking) {
+                {
                     const float tn = fminf(W.tmx, fminf(W.tmy, W.tmz));
                     const float len = fminf(tn, W.d) - W.t;
                     if (len > 0.f) {
@@ -805,7 +882,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                                 pend_len += len;
                             } else {
                                 if (pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
-                                else if (pend_idx >= 0) { unsafeAtomicAdd(grid + pend_idx, (double)pend_len); ev[E_ATOMICS]++; }
+                                else if (pend_idx >= 0) {
+                                    gpush = true;
+                                    gidx = gbase + (uint32_t)pend_idx;
+                                    glen = pend_len;
+                                    ev[E_ATOMICS]++;
+                                }
                                 pend_idx = W.idx;
                                 pend_len = len;
                                 const unsigned lx = (unsigned)(W.ix - P.t0[0]), ly = (unsigned)(W.iy - P.t0[1]),
@@ -815,34 +897,55 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                             }
                         }
                     }
-                    if (tn >= W.d) { walk_done = true; break; }
-                    W.t = tn;
-                    if (W.tmx <= W.tmy && W.tmx <= W.tmz) {
-                        const int s = L.dx > 0.f ? 1 : -1;
-                        W.ix += s;
-                        if ((unsigned)W.ix >= (unsigned)P.nx) { W.inside = false; walk_done = true; break; }
-                        W.idx += s;
-                        W.tmx += f_rcp(fabsf(L.dx) * P.sx);
-                    } else if (W.tmy <= W.tmz) {
-                        const int s = L.dy > 0.f ? 1 : -1;
-                        W.iy += s;
-                        if ((unsigned)W.iy >= (unsigned)P.ny) { W.inside = false; walk_done = true; break; }
-                        W.idx += s * P.nx;
-                        W.tmy += f_rcp(fabsf(L.dy) * P.sy);
+                    if (tn >= W.d) {
+                        walk_done = true;
+                        walking = false;
                     } else {
-                        const int s = L.dz > 0.f ? 1 : -1;
-                        W.iz += s;
-                        if ((unsigned)W.iz >= (unsigned)P.nz) { W.inside = false; walk_done = true; break; }
-                        W.idx += s * P.nx * P.ny;
-                        W.tmz += f_rcp(fabsf(L.dz) * P.sz);
+                        W.t = tn;
+                        if (W.tmx <= W.tmy && W.tmx <= W.tmz) {
+                            const int s = L.dx > 0.f ? 1 : -1;
+                            W.ix += s;
+                            if ((unsigned)W.ix >= (unsigned)P.nx) {
+                                W.inside = false;
+                                walk_done = true;
+                                walking = false;
+                            } else {
+                                W.idx += s;
+                                W.tmx += f_rcp(fabsf(L.dx) * P.sx);
+                            }
+                        } else if (W.tmy <= W.tmz) {
+                            const int s = L.dy > 0.f ? 1 : -1;
+                            W.iy += s;
+                            if ((unsigned)W.iy >= (unsigned)P.ny) {
+                                W.inside = false;
+                                walk_done = true;
+                                walking = false;
+                            } else {
+                                W.idx += s * P.nx;
+                                W.tmy += f_rcp(fabsf(L.dy) * P.sy);
+                            }
+                        } else {
+                            const int s = L.dz > 0.f ? 1 : -1;
+                            W.iz += s;
+                            if ((unsigned)W.iz >= (unsigned)P.nz) {
+                                W.inside = false;
+                                walk_done = true;
+                                walking = false;
+                            } else {
+                                W.idx += s * P.nx * P.ny;
+                                W.tmz += f_rcp(fabsf(L.dz) * P.sz);
+                            }
+                        }
                     }
                 }
-            } else {
-                walk_done = true;  // started outside the grid
             }
+            if (P.jmean) tally_push(Q, gpush, gidx, glen, tbase, lane_id);
         }
 
         // ---- step finished: move, then end of flight (albedo test) or next march step ----
+        bool wpush = false;  // weight_scatter deposit for the global tally
+        uint32_t widx = 0u;
+        float wlen = 0.f;
         if (walk_done) {
             W.active = false;
             const float d = W.d;
@@ -880,7 +983,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                     end_kind = classify_escape(C, L);
                 } else {
                     if (grid) {
-                        unsafeAtomicAdd(grid + (size_t)vx + (size_t)P.nx * ((size_t)vy + (size_t)P.ny * vz), (double)wa);
+                        wpush = true;
+                        widx = gbase + (uint32_t)(vx + P.nx * (vy + P.ny * vz));
+                        wlen = wa;
                         ev[E_ATOMICS]++;
                     }
                     L.nsc++;
@@ -933,17 +1038,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
             }
         }
 
+        if (P.jmean) tally_push(Q, wpush, widx, wlen, tbase, lane_id);
+
         // non-finite guard
         if (end_kind < 0 && L.st != ST_IDLE && !W.active &&
             !(fabsf(L.px) < 1e30f && fabsf(L.py) < 1e30f && fabsf(L.pz) < 1e30f)) {
             ev[E_NAN]++;
             end_kind = END_ERR;
         }
+        bool epush = false;  // the packet's last merged chord for the global tally
+        uint32_t eidx = 0u;
+        float elen = 0.f;
         if (end_kind >= 0) {
             if (pend_li >= 0) {  // flush the merged chord before the lane changes packet/set
                 atomicAdd(s_tile + pend_li, pend_len);
             } else if (pend_idx >= 0) {
-                unsafeAtomicAdd(grid + pend_idx, (double)pend_len);
+                epush = true;
+                eidx = gbase + (uint32_t)pend_idx;
+                elen = pend_len;
                 ev[E_ATOMICS]++;
             }
             pend_idx = -1;
@@ -978,7 +1090,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
             }
             L.st = ST_IDLE;
         }
+        if (P.jmean) tally_push(Q, epush, eidx, elen, tbase, lane_id);
     }
+    if (P.jmean) tally_drain(Q, tbase, lane_id, true);
 
     // ---- flush the tally tile (single-set launches only) into the FP64 grid ----
     if (tile_vox > 0) {
@@ -989,7 +1103,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) p
                 const int lx = i % P.tn[0], r = i / P.tn[0];
                 const int ly = r % P.tn[1], lz = r / P.tn[1];
                 const size_t g = (size_t)(P.t0[0] + lx) + (size_t)P.nx * ((size_t)(P.t0[1] + ly) + (size_t)P.ny * (P.t0[2] + lz));
-                unsafeAtomicAdd(P.jmean + g, (double)v);
+                unsafeAtomicAdd(tbase + g, (double)v);
             }
         }
     }

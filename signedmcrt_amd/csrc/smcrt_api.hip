@@ -66,6 +66,25 @@ __global__ void k_sum_f64(const double* __restrict__ p, size_t n, double* out) {
     if (threadIdx.x == 0) atomicAdd(out, part[0]);
 }
 
+// dst[i] += sum over the partial grids c of src[c * n + i] (per-XCD tallies, fixed order)
+__global__ void k_reduce_copies(double* __restrict__ dst, const double* __restrict__ src, size_t n, int copies) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int c = 0; c < copies; ++c) v += src[(size_t)c * n + i];
+        if (v != 0.0) dst[i] += v;
+    }
+}
+
+// Partial grids per XCD for a tally of n doubles: how many (at least 2, else 0 = tally straight
+// into the grid), within a memory cap.
+static int xgrid_copies(int n_xcc, size_t n, uint32_t flags) {
+    if ((flags & SMCRT_FLAG_SHARED_GRID) || n_xcc < 2 || n == 0) return 0;
+    const size_t cap = (size_t)8 << 30;  // bytes
+    size_t c = cap / (n * sizeof(double));
+    if (c > (size_t)n_xcc) c = (size_t)n_xcc;
+    return c >= 2 ? (int)c : 0;
+}
+
 // SDF values / exact gradients of one region at n points (one point per thread; the region's
 // program is uniform across the grid).
 __global__ void k_evaluate(const DevNode* __restrict__ gnodes, const DevRegion* __restrict__ gregions, int region,
@@ -79,7 +98,7 @@ __global__ void k_evaluate(const DevNode* __restrict__ gnodes, const DevRegion* 
     if (dist) dist[i] = region_eval<float>(nodes, R->first, R->count, V3<float>{x, y, z}, accel);
     if (nrm) {
         // the packet kernel's gradient (lazy through flat folds)
-        const V3<float> g = region_gradient<KM_FULL>(nodes, R, V3<float>{x, y, z}, accel);
+        const V3<float> g = region_gradient<KM_FULL>(nodes, R->first, R->count, R->flat, V3<float>{x, y, z}, accel);
         float inv = 1.0f / sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
         nrm[3 * i] = g.x * inv;
         nrm[3 * i + 1] = g.y * inv;
@@ -157,13 +176,14 @@ struct smcrt_scene {
     // device
     int device = -1;
     int num_cu = 0;
+    int n_xcc = 1;  // XCDs of the device (one FP64 partial grid each)
     bool uploaded = false;
     bool mats_dirty = true;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
         d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold, d_det, d_det_counts, d_accel,
-        d_adj_mats, d_adj_mus;
+        d_adj_mats, d_adj_mus, d_xgrid;
 };
 
 // ---- polygon accelerator (host build) ------------------------------------------------------
@@ -401,6 +421,9 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
     if (rc) return rc;
     const smcrt_grid& g = d->grid;
     if (g.nx < 1 || g.ny < 1 || g.nz < 1) return fail(SMCRT_EINVAL, "grid counts must be >= 1");
+    // the kernel addresses grid elements with 32-bit indices
+    if ((uint64_t)g.nx * (uint64_t)g.ny * (uint64_t)g.nz >= (1ull << 32))
+        return fail(SMCRT_ERANGE, "grid of %d x %d x %d voxels: at most 2^32 - 1 voxels", g.nx, g.ny, g.nz);
     if ((double)g.nx * g.ny * g.nz > 2147483647.0) return fail(SMCRT_ERANGE, "grid has more than 2^31-1 voxels");
     if (!(g.xmax > 0 && g.ymax > 0 && g.zmax > 0)) return fail(SMCRT_EINVAL, "grid half sizes must be > 0");
 
@@ -604,7 +627,7 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
     if (s->device >= 0) {
         (void)hipSetDevice(s->device);
         DevBuf* bufs[] = {&s->d_nodes, &s->d_regions, &s->d_mats, &s->d_region_mat, &s->d_mat_mus, &s->d_work,
-                          &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights,
+                          &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights, &s->d_xgrid,
                           &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum, &s->d_cold, &s->d_det,
                           &s->d_det_counts, &s->d_accel, &s->d_adj_mats, &s->d_adj_mus};
         for (DevBuf* b : bufs) b->release();
@@ -669,6 +692,8 @@ static int ensure_device(smcrt_scene* s) {
         HIPCHECK(hipGetDevice(&dev));
         s->device = dev;
         HIPCHECK(hipDeviceGetAttribute(&s->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (hipDeviceGetAttribute(&s->n_xcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || s->n_xcc < 1)
+            s->n_xcc = 1;
         HIPCHECK(hipStreamCreate(&s->stream));
         HIPCHECK(hipEventCreate(&s->ev0));
         HIPCHECK(hipEventCreate(&s->ev1));
@@ -796,13 +821,15 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
     L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
     const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
-    L.lds = mats;
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
     if (L.block > 256) L.block = 256;
+    // per lane: the wavefront's tally queue (TALLY_QUEUE 8-byte entries per 64 lanes)
+    const size_t queue_lane = sizeof(uint2) * TALLY_QUEUE / 64;
+    L.lds = mats + queue_lane * L.block;
     if (adj) {
         // per lane: 4 FP32 scores per adjoint material; per block: 4 FP64 sums per material
-        const size_t per_lane = (size_t)16 * (size_t)adj_slots, per_block = (size_t)32 * (size_t)adj_slots;
+        const size_t per_lane = (size_t)16 * (size_t)adj_slots + queue_lane, per_block = (size_t)32 * (size_t)adj_slots;
         while (L.block > 64 && mats + per_lane * L.block + per_block > 65536) L.block /= 2;
         L.lds = mats + per_lane * L.block + per_block;
     }
@@ -890,7 +917,8 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_variant(s).fwd, 256, 0) != hipSuccess || bpc < 1)
             bpc = 2;
         const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
-        const long per_block = (long)lds_cu / bpc - mats - 256;
+        const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
+        const long per_block = (long)lds_cu / bpc - mats - queue - 256;
         const int budget = (int)std::max(0L, std::min(16384L, per_block / (long)sizeof(float)));
         if (budget > 0) choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0, s->tile_tn);
     }
@@ -921,6 +949,19 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     }
     P.jmean = d_jmean;
     P.grid_stride = 0;
+    P.xgrid = nullptr;
+    P.xstride = 0;
+    P.xcopies = 0;
+    const size_t nvox = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+    const int copies = d_jmean ? xgrid_copies(s->n_xcc, nvox, o->flags) : 0;
+    if (copies) {
+        int rc = s->d_xgrid.ensure((size_t)copies * nvox * sizeof(double));
+        if (rc) return rc;
+        P.xgrid = s->d_xgrid.as<double>();
+        P.xstride = nvox;
+        P.xcopies = copies;
+        k_fill_f64<<<(unsigned)fill_blocks((size_t)copies * nvox), 256, 0, stream>>>(P.xgrid, (size_t)copies * nvox);
+    }
     P.seed_lo = (uint32_t)o->seed;
     P.seed_hi = (uint32_t)(o->seed >> 32);
     P.total = o->n_packets;
@@ -976,6 +1017,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     HIPCHECK(launch_packet_kernel(s, adj, sh, stream, P));
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(s->ev1, stream));
+    if (copies) k_reduce_copies<<<(unsigned)fill_blocks(nvox), 256, 0, stream>>>(d_jmean, P.xgrid, nvox, copies);
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
     return SMCRT_OK;
@@ -1192,8 +1234,9 @@ struct smcrt_batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    DevBuf d_mats, d_grids, d_stats, d_info, d_work, d_cold;
+    DevBuf d_mats, d_grids, d_stats, d_info, d_work, d_cold, d_xgrid;
     size_t total_doubles = 0, total_stats = 0;
+    int copies = 0;  // per-XCD partial grids (0: none)
 };
 
 extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt_batch_set* sets,
@@ -1254,14 +1297,20 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         nstats += g.sets.size() * NOUT + NEV;
         ninfo += 2 * g.sets.size() + 1;
     }
+    if (ndoubles >= (1ull << 32)) {
+        smcrt_batch_destroy(b);
+        return fail(SMCRT_ERANGE, "a batch holds %zu grid voxels; at most 2^32 - 1 (32-bit tally indices)", ndoubles);
+    }
     b->total_doubles = ndoubles;
     b->total_stats = nstats;
+    b->copies = xgrid_copies(scenes[0]->n_xcc, ndoubles, shape ? shape->flags : 0u);
     HIPCHECK(hipSetDevice(b->device));
     int rc;
     if ((rc = b->d_mats.ensure(nmats * sizeof(DevMat))) || (rc = b->d_grids.ensure(ndoubles * sizeof(double))) ||
         (rc = b->d_stats.ensure(nstats * sizeof(unsigned long long))) ||
         (rc = b->d_info.ensure(ninfo * sizeof(unsigned long long))) ||
-        (rc = b->d_work.ensure(b->groups.size() * sizeof(unsigned long long)))) {
+        (rc = b->d_work.ensure(b->groups.size() * sizeof(unsigned long long))) ||
+        (b->copies && (rc = b->d_xgrid.ensure((size_t)b->copies * ndoubles * sizeof(double))))) {
         smcrt_batch_destroy(b);
         return rc;
     }
@@ -1289,6 +1338,9 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         P.mats = b->d_mats.as<DevMat>() + mat_off;
         P.jmean = b->d_grids.as<double>() + g.grid_off;
         P.grid_stride = (unsigned long long)s->grid.nx * s->grid.ny * s->grid.nz;
+        P.xgrid = b->copies ? b->d_xgrid.as<double>() + g.grid_off : nullptr;
+        P.xstride = b->copies ? ndoubles : 0;
+        P.xcopies = b->copies;
         P.seed_lo = (uint32_t)seed;
         P.seed_hi = (uint32_t)(seed >> 32);
         P.total = acc;
@@ -1324,6 +1376,9 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
     // one packet-kernel node per geometry group
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     k_fill_f64<<<(unsigned)fill_blocks(ndoubles), 256, 0, b->stream>>>(b->d_grids.as<double>(), ndoubles);
+    if (b->copies)
+        k_fill_f64<<<(unsigned)fill_blocks((size_t)b->copies * ndoubles), 256, 0, b->stream>>>(
+            b->d_xgrid.as<double>(), (size_t)b->copies * ndoubles);
     k_fill_u64<<<(unsigned)fill_blocks(nstats), 256, 0, b->stream>>>(b->d_stats.as<unsigned long long>(), nstats);
     k_fill_u64<<<1, 64, 0, b->stream>>>(b->d_work.as<unsigned long long>(), b->groups.size());
     for (BatchGroup& g : b->groups) {
@@ -1336,6 +1391,10 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
             return fail(SMCRT_EDEVICE, "packet kernel launch during capture failed: %s", hipGetErrorString(le));
         }
     }
+    if (b->copies)
+        k_reduce_copies<<<(unsigned)fill_blocks(ndoubles), 256, 0, b->stream>>>(b->d_grids.as<double>(),
+                                                                                b->d_xgrid.as<double>(), ndoubles,
+                                                                                b->copies);
     hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
     if (ce != hipSuccess) {
         smcrt_batch_destroy(b);
@@ -1395,6 +1454,7 @@ extern "C" int smcrt_batch_destroy(smcrt_batch* b) {
     b->d_info.release();
     b->d_work.release();
     b->d_cold.release();
+    b->d_xgrid.release();
     delete b;
     return SMCRT_OK;
 }

@@ -339,3 +339,24 @@ def test_kernel_variants_agree(gpu, name, monkeypatch):
     m = b.jmean > 1e-3 * b.jmean.max()
     assert np.max(np.abs(a.jmean[m] / b.jmean[m] - 1)) < 1e-4
     np.testing.assert_allclose(ga, gb, rtol=1e-6, atol=1e-12 * np.abs(gb).max())
+
+
+@pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels"])
+def test_per_xcd_partial_grids_match_shared_grid(gpu, name):
+    """One FP64 partial grid per XCD (the default), summed after the launch, against every block
+    tallying into the one grid (SMCRT_FLAG_SHARED_GRID): same packets, same counters, the same
+    fluence to the FP32 tile's accumulation order; a second launch starts from cleared partial
+    grids (no carry-over)."""
+    from signedmcrt_amd import _abi as A
+    kw = {"nxyz": 64} if name == "skin_vessels" else {}
+    sc, _ = scenes.build(name, **kw)
+    eng = Engine(sc)
+    a = eng.run(300_000, seed=41)
+    a2 = eng.run(300_000, seed=41)
+    b = eng.run(300_000, seed=41, flags=A.FLAG_SHARED_GRID)
+    for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface", "scatters", "ev_crossings", "ev_atomics"):
+        assert a.stats[k] == b.stats[k] == a2.stats[k], k
+    m = b.jmean > 1e-3 * b.jmean.max()
+    for x in (a, a2):
+        assert abs(x.jmean.sum() / b.jmean.sum() - 1) < 1e-6
+        assert np.max(np.abs(x.jmean[m] / b.jmean[m] - 1)) < 1e-4

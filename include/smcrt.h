@@ -207,7 +207,7 @@ typedef struct smcrt_run_opts {
                               shape and shards/CRN runs line up                              */
     uint32_t flags;        /* SMCRT_FLAG_*                                                   */
     int32_t max_bounces;   /* bounce cap, 0 -> 1000 (src/inttau2.f90:177)                    */
-    int32_t block_size;    /* 0 -> library default (64..1024, multiple of 64)                */
+    int32_t block_size;    /* 0 -> library default (64..256, multiple of 64)                 */
     int32_t blocks_per_cu; /* 0 -> library default                                           */
     int32_t chunk;         /* packets a wavefront claims per refill, 0 -> default            */
     int32_t trace_packets; /* record the first `trace_packets` packets' events (0 = off)    */
@@ -217,7 +217,10 @@ typedef struct smcrt_run_opts {
                               least this many are idle (1 = at once; 0 -> default)            */
     uint32_t grad_mask_lo; /* smcrt_grad: materials whose derivatives are computed, bit m =   */
     uint32_t grad_mask_hi; /* material m (0 and 0: every material); at most 16 set            */
-    int32_t reserved;
+    int32_t waves;         /* resident wavefronts per SIMD the packet kernel is built for: 2 or
+                              3 (0 -> 3).  3 exists for the kind-specialised forward kernels
+                              (fewer registers, a few spilled; faster where latency dominates
+                              over the tally); other launches run the 2-wave build            */
 } smcrt_run_opts;
 
 /* How a packet ended.  A+R+T+K = launched. */

@@ -89,7 +89,7 @@ class RunOpts(C.Structure):
                 ("blocks_per_cu", C.c_int32), ("chunk", C.c_int32),
                 ("trace_packets", C.c_int32), ("trace_events", C.c_int32),
                 ("walk_cap", C.c_int32), ("refill_min", C.c_int32),
-                ("grad_mask_lo", C.c_uint32), ("grad_mask_hi", C.c_uint32), ("reserved", C.c_int32)]
+                ("grad_mask_lo", C.c_uint32), ("grad_mask_hi", C.c_uint32), ("waves", C.c_int32)]
 
 
 STAT_COUNTERS = ["launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",

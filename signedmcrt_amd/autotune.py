@@ -9,6 +9,7 @@ Knobs (`SPACE`):
   refill_min  regeneration policy: a wavefront refills its idle lanes once at least this many
               are idle (1 = at once)
   tile        LDS-privatised FP32 tally tile on/off
+  waves       resident wavefronts per SIMD the kernel is built for (2, or 3: fewer registers)
 
 Every probe launches the same packets (fixed seed and first packet id).  Each packet draws from
 its own (seed, id) random stream, so the transported work is identical for every shape: the
@@ -38,8 +39,9 @@ SPACE: Dict[str, Tuple] = {
     "walk_cap": (1, 2, 4, 8, 16),
     "refill_min": (1, 8, 16, 32),
     "tile": (True, False),
+    "waves": (2, 3),
 }
-DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True}
+DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "waves": 3}
 # counters that must not change with the launch shape
 INVARIANT = ("launched", "absorbed", "escaped_r", "escaped_t", "killed_surface", "killed_bounce",
              "scatters", "ev_steps", "ev_crossings", "ev_interfaces")
@@ -47,7 +49,7 @@ INVARIANT = ("launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",
 
 def run_kwargs(shape: Dict) -> Dict:
     """Engine.run / run_device keywords for a shape dict."""
-    kw = {k: int(shape[k]) for k in ("block", "chunk", "walk_cap", "refill_min") if k in shape}
+    kw = {k: int(shape[k]) for k in ("block", "chunk", "walk_cap", "refill_min", "waves") if k in shape}
     kw["flags"] = 0 if shape.get("tile", True) else A.FLAG_NO_LDS_TILE
     return kw
 

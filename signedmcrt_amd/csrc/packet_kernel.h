@@ -527,10 +527,10 @@ SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
     return true;
 }
 
-template <bool ADJ, uint32_t KM>
-// at least 2 resident waves per SIMD (at most 256 VGPRs; the general variant would otherwise
-// take more registers for 1)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) packet_kernel(KParams P) {
+// WAVES: resident waves per SIMD the registers are allocated for (2: at most 256 VGPRs, 3: at most
+// 168, with some spilled in the specialised forward kernels)
+template <bool ADJ, uint32_t KM, int WAVES = 2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     CDevNode* const nodes = (CDevNode*)P.nodes;
     CKCold* const C = (CKCold*)P.cold;

@@ -170,9 +170,9 @@ struct smcrt_scene {
     std::vector<smcrt_detector> dets;
     std::vector<DevDet> ddets;
     size_t det_bins = 0;
-    // LDS tally tile (voxel box), chosen from a pilot run on first use
+    // LDS tally tile (voxel box) for the 2- and 3-wave kernels, chosen from a pilot run on first use
     bool tile_ready = false;
-    int tile_t0[3] = {0, 0, 0}, tile_tn[3] = {0, 0, 0};
+    int tile_t0[2][3] = {{0, 0, 0}, {0, 0, 0}}, tile_tn[2][3] = {{0, 0, 0}, {0, 0, 0}};
     // device
     int device = -1;
     int num_cu = 0;
@@ -770,7 +770,7 @@ static void fill_common(const smcrt_scene* s, KParams& P, KCold& Cc) {
 }
 
 struct LaunchShape {
-    int block, blocks, chunk, walk_cap, refill_min;
+    int block, blocks, chunk, walk_cap, refill_min, waves;
     size_t lds;
 };
 
@@ -786,12 +786,15 @@ struct KernelVariant {
     uint32_t mask;
     const char* name;
     PacketKernelFn fwd, adj;
+    PacketKernelFn fwd3;  // forward kernel built for 3 resident waves per SIMD (nullptr: none)
 };
 static const KernelVariant kVariants[] = {
-    {KM_BASIC, "basic", packet_kernel<false, KM_BASIC>, packet_kernel<true, KM_BASIC>},
-    {KM_LAYERS, "layers", packet_kernel<false, KM_LAYERS>, packet_kernel<true, KM_LAYERS>},
-    {KM_EXTRUDED, "extruded", packet_kernel<false, KM_EXTRUDED>, packet_kernel<true, KM_EXTRUDED>},
-    {KM_FULL, "full", packet_kernel<false, KM_FULL>, packet_kernel<true, KM_FULL>},
+    {KM_BASIC, "basic", packet_kernel<false, KM_BASIC>, packet_kernel<true, KM_BASIC>, packet_kernel<false, KM_BASIC, 3>},
+    {KM_LAYERS, "layers", packet_kernel<false, KM_LAYERS>, packet_kernel<true, KM_LAYERS>,
+     packet_kernel<false, KM_LAYERS, 3>},
+    {KM_EXTRUDED, "extruded", packet_kernel<false, KM_EXTRUDED>, packet_kernel<true, KM_EXTRUDED>,
+     packet_kernel<false, KM_EXTRUDED, 3>},
+    {KM_FULL, "full", packet_kernel<false, KM_FULL>, packet_kernel<true, KM_FULL>, nullptr},
 };
 
 static const KernelVariant& kernel_variant(const smcrt_scene* s) {
@@ -803,11 +806,26 @@ static const KernelVariant& kernel_variant(const smcrt_scene* s) {
     return kVariants[nv - 1];
 }
 
+// The kernel a launch runs: the scene's variant, adjoint or forward, and the 3-wave build when
+// asked for and available.
+static PacketKernelFn packet_kernel_for(const smcrt_scene* s, bool adj, int waves) {
+    const KernelVariant& v = kernel_variant(s);
+    if (adj) return v.adj;
+    return (waves == 3 && v.fwd3) ? v.fwd3 : v.fwd;
+}
+
+// Resident waves per SIMD a launch runs with: 3 (the default) where a 3-wave build exists,
+// else 2.
+static int launch_waves(const smcrt_scene* s, const smcrt_run_opts* o, bool adj) {
+    const int w = (o && o->waves > 0) ? o->waves : 3;
+    return (w == 3 && !adj && kernel_variant(s).fwd3) ? 3 : 2;
+}
+
 static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const LaunchShape& sh, hipStream_t stream,
                                        KParams& P) {
-    const KernelVariant& v = kernel_variant(s);
     void* args[] = {&P};
-    return hipLaunchKernel((const void*)(adj ? v.adj : v.fwd), dim3(sh.blocks), dim3(sh.block), args, sh.lds, stream);
+    return hipLaunchKernel((const void*)packet_kernel_for(s, adj, sh.waves), dim3(sh.blocks), dim3(sh.block), args,
+                           sh.lds, stream);
 }
 
 // Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
@@ -820,6 +838,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
     L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
+    L.waves = launch_waves(s, o, adj);
     const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
@@ -836,8 +855,8 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once
         int active = 0;
-        const KernelVariant& v = kernel_variant(s);
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, adj ? v.adj : v.fwd, L.block, L.lds);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel_for(s, adj, L.waves), L.block,
+                                                                   L.lds);
         if (e == hipSuccess && active > 0) bpc = active;
     }
     L.blocks = s->num_cu * bpc;
@@ -909,18 +928,23 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
     std::vector<double> h(nv);
     if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
         // the tile takes the LDS left per 256-lane block at the kernel's register occupancy
-        // (2 or 3 blocks per CU), at most 64 KiB of FP32
-        int lds_cu = 0, bpc = 0;
+        // (2 or 3 blocks per CU for the 2- and 3-wave builds), at most 64 KiB of FP32
+        int lds_cu = 0;
         if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
             lds_cu <= 0)
             lds_cu = 160 * 1024;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_variant(s).fwd, 256, 0) != hipSuccess || bpc < 1)
-            bpc = 2;
-        const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
-        const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
-        const long per_block = (long)lds_cu / bpc - mats - queue - 256;
-        const int budget = (int)std::max(0L, std::min(16384L, per_block / (long)sizeof(float)));
-        if (budget > 0) choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0, s->tile_tn);
+        for (int w = 2; w <= 3; ++w) {
+            int bpc = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, packet_kernel_for(s, false, w), 256, 0) != hipSuccess ||
+                bpc < 1)
+                bpc = w;
+            const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
+            const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
+            const long per_block = (long)lds_cu / bpc - mats - queue - 256;
+            const int budget = (int)std::max(0L, std::min(16384L, per_block / (long)sizeof(float)));
+            if (budget > 0)
+                choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0[w - 2], s->tile_tn[w - 2]);
+        }
     }
     (void)hipFree(pilot);
     return rc;
@@ -986,7 +1010,8 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     Cc.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
     int tile_vox = 0;
     if (use_tile) {
-        for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[a]; P.tn[a] = s->tile_tn[a]; }
+        const int wi = launch_waves(s, o, adj) - 2;
+        for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }
     LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat);

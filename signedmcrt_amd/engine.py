@@ -58,12 +58,12 @@ def _fptr(a: np.ndarray):
 def run_opts(n: int, seed: int = 123456789, first_packet: int = 0, flags: int = 0,
              block: int = 0, blocks_per_cu: int = 0, chunk: int = 0, max_bounces: int = 0,
              trace_packets: int = 0, trace_events: int = 0, walk_cap: int = 0,
-             refill_min: int = 0) -> A.RunOpts:
+             refill_min: int = 0, waves: int = 0) -> A.RunOpts:
     o = A.RunOpts()
     o.n_packets, o.seed, o.first_packet, o.flags = int(n), int(seed), int(first_packet), int(flags)
     o.block_size, o.blocks_per_cu, o.chunk, o.max_bounces = block, blocks_per_cu, chunk, max_bounces
     o.trace_packets, o.trace_events, o.walk_cap = trace_packets, trace_events, walk_cap
-    o.refill_min = refill_min
+    o.refill_min, o.waves = refill_min, waves
     return o
 
 

@@ -87,7 +87,7 @@ module smcrt
         integer(c_int64_t) :: n_packets = 0, seed = 123456789, first_packet = 0
         integer(c_int32_t) :: flags = 0, max_bounces = 0, block_size = 0, blocks_per_cu = 0
         integer(c_int32_t) :: chunk = 0, trace_packets = 0, trace_events = 0, walk_cap = 0
-        integer(c_int32_t) :: refill_min = 0, grad_mask_lo = 0, grad_mask_hi = 0, reserved = 0
+        integer(c_int32_t) :: refill_min = 0, grad_mask_lo = 0, grad_mask_hi = 0, waves = 0
     end type smcrt_run_opts
 
     type, bind(C) :: smcrt_stats

@@ -786,16 +786,20 @@ struct KernelVariant {
     uint32_t mask;
     const char* name;
     PacketKernelFn fwd, adj;
-    PacketKernelFn fwd3;  // forward kernel built for 3 resident waves per SIMD (nullptr: none)
+    PacketKernelFn fwd3, adj3;  // built for 3 resident waves per SIMD (nullptr: none)
 };
+#define SMCRT_VARIANT(KM, NAME)                                                                 \
+    {                                                                                           \
+        KM, NAME, packet_kernel<false, KM>, packet_kernel<true, KM>, packet_kernel<false, KM, 3>, \
+            packet_kernel<true, KM, 3>                                                          \
+    }
 static const KernelVariant kVariants[] = {
-    {KM_BASIC, "basic", packet_kernel<false, KM_BASIC>, packet_kernel<true, KM_BASIC>, packet_kernel<false, KM_BASIC, 3>},
-    {KM_LAYERS, "layers", packet_kernel<false, KM_LAYERS>, packet_kernel<true, KM_LAYERS>,
-     packet_kernel<false, KM_LAYERS, 3>},
-    {KM_EXTRUDED, "extruded", packet_kernel<false, KM_EXTRUDED>, packet_kernel<true, KM_EXTRUDED>,
-     packet_kernel<false, KM_EXTRUDED, 3>},
-    {KM_FULL, "full", packet_kernel<false, KM_FULL>, packet_kernel<true, KM_FULL>, nullptr},
+    SMCRT_VARIANT(KM_BASIC, "basic"),
+    SMCRT_VARIANT(KM_LAYERS, "layers"),
+    SMCRT_VARIANT(KM_EXTRUDED, "extruded"),
+    {KM_FULL, "full", packet_kernel<false, KM_FULL>, packet_kernel<true, KM_FULL>, nullptr, nullptr},
 };
+#undef SMCRT_VARIANT
 
 static const KernelVariant& kernel_variant(const smcrt_scene* s) {
     const size_t nv = sizeof kVariants / sizeof kVariants[0];
@@ -810,15 +814,16 @@ static const KernelVariant& kernel_variant(const smcrt_scene* s) {
 // asked for and available.
 static PacketKernelFn packet_kernel_for(const smcrt_scene* s, bool adj, int waves) {
     const KernelVariant& v = kernel_variant(s);
-    if (adj) return v.adj;
-    return (waves == 3 && v.fwd3) ? v.fwd3 : v.fwd;
+    const PacketKernelFn k3 = adj ? v.adj3 : v.fwd3;
+    return (waves == 3 && k3) ? k3 : (adj ? v.adj : v.fwd);
 }
 
 // Resident waves per SIMD a launch runs with: 3 (the default) where a 3-wave build exists,
 // else 2.
 static int launch_waves(const smcrt_scene* s, const smcrt_run_opts* o, bool adj) {
     const int w = (o && o->waves > 0) ? o->waves : 3;
-    return (w == 3 && !adj && kernel_variant(s).fwd3) ? 3 : 2;
+    const KernelVariant& v = kernel_variant(s);
+    return (w == 3 && (adj ? v.adj3 : v.fwd3)) ? 3 : 2;
 }
 
 static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const LaunchShape& sh, hipStream_t stream,

@@ -220,8 +220,8 @@ SMCRT_DEV void scatter(const KParams& P, Lane& L, float hgg, float g2) {
     if (hgg == 0.f) {
         cost = 2.f * rnd(L, P) - 1.f;
     } else {
-        float temp = (1.f - g2) / (1.f - hgg + 2.f * hgg * rnd(L, P));
-        cost = (1.f + g2 - temp * temp) / (2.f * hgg);
+        float temp = f_div(1.f - g2, 1.f - hgg + 2.f * hgg * rnd(L, P));
+        cost = f_div(1.f + g2 - temp * temp, 2.f * hgg);
         cost = fminf(fmaxf(cost, -1.f), 1.f);
     }
     float sint = f_sqrt(fmaxf(1.f - cost * cost, 0.f));
@@ -250,12 +250,12 @@ SMCRT_DEV void scatter(const KParams& P, Lane& L, float hgg, float g2) {
 SMCRT_DEV float fresnel(float ix, float iy, float iz, float nx, float ny, float nz, float n1, float n2) {
     float costt = fminf(fabsf(ix * nx + iy * ny + iz * nz), 1.f);
     float sintt = f_sqrt(fmaxf(1.f - costt * costt, 0.f));
-    float sint2 = n1 / n2 * sintt;
+    float sint2 = f_div(n1, n2) * sintt;
     if (sint2 > 1.f) return 1.f;
     if (costt == 1.f) return 0.f;
     float cost2 = f_sqrt(fmaxf(1.f - sint2 * sint2, 0.f));
-    float f1 = (n1 * costt - n2 * cost2) / (n1 * costt + n2 * cost2);
-    float f2 = (n1 * cost2 - n2 * costt) / (n1 * cost2 + n2 * costt);
+    float f1 = f_div(n1 * costt - n2 * cost2, n1 * costt + n2 * cost2);
+    float f2 = f_div(n1 * cost2 - n2 * costt, n1 * cost2 + n2 * costt);
     return 0.5f * (f1 * f1 + f2 * f2);
 }
 
@@ -284,16 +284,18 @@ SMCRT_DEV T eval_lane_region(CDevNode* nodes, CDevRegion* regions, bool active, 
 template <class T, uint32_t KM>
 SMCRT_DEV T prim_node(CDevNode* nodes, int i, const V3<T>& p, const float* __restrict__ accel) {
     CDevNode* n = nodes + __builtin_amdgcn_readfirstlane(i);
-    float pp[8], tt[12];
+    float pp[8], qq[8], tt[12];
 #pragma unroll
     for (int j = 0; j < 8; ++j) pp[j] = n->p[j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qq[j] = n->q[j];
     V3<T> q = p;
     if (!n->ident) {
 #pragma unroll
         for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
         q = xform<T>(tt, p);
     }
-    return prim_eval<T, KM>(n->kind, pp, q, accel);
+    return prim_eval<T, KM>(n->kind, pp, qq, q, accel);
 }
 
 SMCRT_DEV V3<float> dual_grad(const Dual& d) { return V3<float>{d.x, d.y, d.z}; }
@@ -317,7 +319,11 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
     if (!flat) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
     const int nch = (count + 1) / 2;
     const bool smooth = KM_HAS(KM, KM_CSG(2)) && flat == 2;
-    const float k = smooth ? nodes[__builtin_amdgcn_readfirstlane(first + 2)].k : 0.f;
+    const int ki = __builtin_amdgcn_readfirstlane(first + 2);  // the fold's first operator node
+    const float k = smooth ? nodes[ki].k : 0.f;
+    float kq[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) kq[j] = smooth ? nodes[ki].q[j] : 0.f;
     // pass 1: index of the last child that wins outright
     float a = prim_node<float, KM>(nodes, first, p, accel);
     int last = 0;
@@ -325,7 +331,7 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
         const float c = prim_node<float, KM>(nodes, first + 2 * j - 1, p, accel);
         const bool wins = smooth ? (a - c >= k) : (c < a);
         last = wins ? j : last;
-        a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k) : fminf(a, c);
+        a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k, kq) : fminf(a, c);
     }
     // pass 2: forward-mode from that child on
     int li = last == 0 ? first : first + 2 * last - 1;
@@ -352,7 +358,7 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
         const float c = prim_node<float, KM>(nodes, ni, p, accel);
         float alpha, beta;
         if (smooth) {
-            const float h = fmaxf(k - fabsf(a - c), 0.f) * (1.0f / k), h2 = 0.5f * h * h;
+            const float h = fmaxf(k - fabsf(a - c), 0.f) * kq[0], h2 = 0.5f * h * h;
             alpha = c < a ? h2 : 1.f - h2;
             beta = c < a ? 1.f - h2 : h2;
         } else {
@@ -362,7 +368,7 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
         V3<float> gc{0.f, 0.f, 0.f};
         if (beta != 0.f) gc = dual_grad(prim_node<Dual, KM>(nodes, ni, pd, accel));
         g = V3<float>{alpha * g.x + beta * gc.x, alpha * g.y + beta * gc.y, alpha * g.z + beta * gc.z};
-        a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k) : fminf(a, c);
+        a = smooth ? csg_eval<float, KM>(SMCRT_SMOOTH_UNION, a, c, k, kq) : fminf(a, c);
     }
     return g;
 }
@@ -813,7 +819,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
                     L.dx -= c * nx; L.dy -= c * ny; L.dz -= c * nz;
                 } else {
-                    float eta = n1 / n2;
+                    float eta = f_div(n1, n2);
                     float c1 = nx * L.dx + ny * L.dy + nz * L.dz;
                     if (c1 < 0.f) c1 = -c1;
                     else { nx = -nx; ny = -ny; nz = -nz; }
@@ -844,7 +850,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             const float t = dmin * kap;
             W.flight_over = !(L.taurun + t <= L.tau);
             float d = dmin;
-            if (W.flight_over) d = (L.tau - L.taurun) / kap;
+            if (W.flight_over) d = f_div(L.tau - L.taurun, kap);
             else L.taurun += t;
             if (L.ghost) {  // outside the grid: move without a walk (nothing to tally)
                 move(L, d);

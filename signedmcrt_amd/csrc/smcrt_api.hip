@@ -186,6 +186,27 @@ struct smcrt_scene {
         d_adj_mats, d_adj_mus, d_xgrid;
 };
 
+// Per-node constants the device would otherwise recompute (in VALU) at every evaluation, from
+// the FP64 parameters: segment/capsule/cylinder b - a, |b - a|^2 and its reciprocal, r |b - a|^2
+// and |b - a|^2 / 2; SmoothUnion 1/k and k/6.
+static void derive_node(DevNode& dn, const smcrt_node& n) {
+    for (int j = 0; j < 8; ++j) dn.q[j] = 0.f;
+    if (n.op == SMCRT_OP_PRIM && (n.kind == SMCRT_SEGMENT || n.kind == SMCRT_CAPSULE || n.kind == SMCRT_CYLINDER)) {
+        const double bx = n.p[3] - n.p[0], by = n.p[4] - n.p[1], bz = n.p[5] - n.p[2];
+        const double baba = bx * bx + by * by + bz * bz;
+        dn.q[0] = (float)bx;
+        dn.q[1] = (float)by;
+        dn.q[2] = (float)bz;
+        dn.q[3] = (float)baba;
+        dn.q[4] = (float)(1.0 / baba);
+        dn.q[5] = (float)(n.p[6] * baba);
+        dn.q[6] = (float)(0.5 * baba);
+    } else if (n.op == SMCRT_OP_CSG && n.kind == SMCRT_SMOOTH_UNION) {
+        dn.q[0] = (float)(1.0 / n.k);
+        dn.q[1] = (float)(n.k / 6.0);
+    }
+}
+
 // ---- polygon accelerator (host build) ------------------------------------------------------
 static double seg_dist(double px, double py, double ax, double ay, double ex, double ey) {
     const double ee = ex * ex + ey * ey;
@@ -489,6 +510,7 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
             dn.kind = n.kind;
             dn.k = (float)n.k;
             for (int j = 0; j < 8; ++j) dn.p[j] = (float)n.p[j];
+            derive_node(dn, n);
             if (n.op == SMCRT_OP_PRIM && n.kind == SMCRT_POLYGON) {
                 const double f = n.p[0], c = n.p[1];
                 if (!(f >= 0 && c >= 3 && f == std::floor(f) && c == std::floor(c) && f + c <= d->n_poly_vertices) ||

@@ -24,7 +24,7 @@ struct alignas(16) DevNode {
     float k;      // CSG smoothing
     float p[8];
     float t[12];
-    float pad[8];
+    float q[8];   // constants derived from p and k on the host (see derive_node in smcrt_api.hip)
 };
 static_assert(sizeof(DevNode) == 128, "DevNode must stay 32 dwords");
 
@@ -253,7 +253,8 @@ SMCRT_DEV T polygon_eval(const float* __restrict__ A, int base, const V3<T>& p) 
 // SDF primitives (src/sdfs/sdfs.f90:494-735)
 // ------------------------------------------------------------------------------------------
 template <class T, uint32_t KM = KM_FULL>
-SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, const float* __restrict__ accel) {
+SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const float* __restrict__ Q, const V3<T>& p,
+                      const float* __restrict__ accel) {
     // a kind outside the mask is never dispatched (the host picks a covering variant): its case
     // compiles to a break
     switch (kind) {
@@ -269,13 +270,13 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
         return g_sqrt(a * a + p.y * p.y) - P[1];
     }
     case 4: if constexpr (!KM_HAS(KM, KM_PRIM(4))) break; {  // cylinder(a, b, r)
-        float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
-        float baba = bax * bax + bay * bay + baz * baz;
+        // Q: b - a, |b - a|^2, 1 / |b - a|^2, r |b - a|^2, |b - a|^2 / 2
+        const float bax = Q[0], bay = Q[1], baz = Q[2], baba = Q[3], hb = Q[6];
         V3<T> pa = vsubc(p, P[0], P[1], P[2]);
         T paba = pa.x * bax + pa.y * bay + pa.z * baz;
         V3<T> w{pa.x * baba - paba * bax, pa.y * baba - paba * bay, pa.z * baba - paba * baz};
-        T x = vlen(w) - P[6] * baba;
-        T y = g_abs(paba - baba * 0.5f) - baba * 0.5f;
+        T x = vlen(w) - Q[5];
+        T y = g_abs(paba - hb) - hb;
         T x2 = x * x;
         T y2 = (y * y) * baba;
         float xv = val(x), yv = val(y);
@@ -285,7 +286,7 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
         else if (xv > 0.f) d = x2;
         else if (yv > 0.f) d = y2;
         else d = g_const<T>(0.f);
-        T r = g_sqrt(g_abs(d)) * (1.0f / baba);
+        T r = g_sqrt(g_abs(d)) * Q[4];
         return g_neg_if(r, val(d) < 0.f);
     }
     case 5: if constexpr (!KM_HAS(KM, KM_PRIM(5))) break; {  // triprism(h1, h2)
@@ -294,10 +295,9 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
     }
     case 6:    // segment (tube thickness 0.1)
     case 7: if constexpr (!KM_HAS(KM, KM_PRIM(6) | KM_PRIM(7))) break; {  // capsule(a, b, r)
-        float bax = P[3] - P[0], bay = P[4] - P[1], baz = P[5] - P[2];
-        float baba = bax * bax + bay * bay + baz * baz;
+        const float bax = Q[0], bay = Q[1], baz = Q[2];  // Q: b - a, |b - a|^2, 1 / |b - a|^2
         V3<T> pa = vsubc(p, P[0], P[1], P[2]);
-        T h = g_clamp01((pa.x * bax + pa.y * bay + pa.z * baz) * (1.0f / baba));
+        T h = g_clamp01((pa.x * bax + pa.y * bay + pa.z * baz) * Q[4]);
         V3<T> w{pa.x - h * bax, pa.y - h * bay, pa.z - h * baz};
         return vlen(w) - (kind == 6 ? 0.1f : P[6]);
     }
@@ -343,14 +343,15 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const V3<T>& p, con
 }
 
 // CSG operators (src/sdfs/sdfModifiers.f90:412-475)
+// Q: 1 / k, k / 6 (SmoothUnion)
 template <class T, uint32_t KM = KM_FULL>
-SMCRT_DEV T csg_eval(int kind, const T& d1, const T& d2, float k) {
+SMCRT_DEV T csg_eval(int kind, const T& d1, const T& d2, float k, const float* __restrict__ Q) {
     switch (kind) {
     case 1: if constexpr (KM_HAS(KM, KM_CSG(1))) return g_min(d1, d2);
         break;
     case 2: if constexpr (KM_HAS(KM, KM_CSG(2))) {
-        T h = g_max0(k - g_abs(d1 - d2)) * (1.0f / k);
-        return g_min(d1, d2) - h * h * h * (k * (1.0f / 6.0f));
+        T h = g_max0(k - g_abs(d1 - d2)) * Q[0];
+        return g_min(d1, d2) - h * h * h * Q[1];
     }
         break;
     case 3: if constexpr (KM_HAS(KM, KM_CSG(3))) return g_max(-d1, d2);
@@ -424,9 +425,11 @@ SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, 
         CDevNode* n = nodes + first + i;
         const int op = n->op;
         const int kind = n->kind;
-        float pp[8], tt[12];
+        float pp[8], qq[8], tt[12];
 #pragma unroll
         for (int j = 0; j < 8; ++j) pp[j] = n->p[j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) qq[j] = n->q[j];
         if (op == 1) {  // PRIM
             V3<T> q = q0;
             if (!n->ident) {
@@ -434,11 +437,11 @@ SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, 
                 for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
                 q = xform<T>(tt, q0);
             }
-            T d = prim_eval<T, KM>(kind, pp, q, accel);
+            T d = prim_eval<T, KM>(kind, pp, qq, q, accel);
             v3 = v2; v2 = v1; v1 = v0; v0 = d;
         } else if (op == 2) {  // CSG: d1 = v1, d2 = v0
             if constexpr (KM_HAS(KM, KM_ANY_CSG)) {
-                T d = csg_eval<T, KM>(kind, v1, v0, n->k);
+                T d = csg_eval<T, KM>(kind, v1, v0, n->k, qq);
                 v0 = d; v1 = v2; v2 = v3;
             }
         } else if (op == 3) {  // MOD_BEGIN

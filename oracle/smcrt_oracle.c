@@ -72,7 +72,24 @@ static void rng_init(orc_rng* g, uint64_t seed, uint64_t id) {
     g->idx = 4;
 }
 
-/* uniform in (0,1): an odd multiple of 2^-24, exactly representable in FP32 and FP64 */
+static double u_of(uint32_t x) { return (double)(((x >> 9) << 1) | 1u) * (1.0 / 16777216.0); }
+
+/* The packet loop draws one Philox block per event, u[0..3], whatever it uses of it:
+ *   emission       u[0], u[1] the emitter's draws, u[2] the first flight's optical depth;
+ *   flight end     u[0] the albedo test (weight estimator: the roulette draw), u[1] cos theta,
+ *                  u[2] the azimuth, u[3] the next flight's optical depth;
+ *   Fresnel event  u[0] reflect or refract, u[1] the resampled optical depth.
+ * (The reference draws from one compiler RNG stream per thread; its bitstream is not
+ * reproducible, so the mapping of draws to the stream is this build's own.) */
+static void rng_block(orc_rng* g, double u[4]) {
+    uint32_t c[4] = {g->block, g->id_lo, g->id_hi, 0u};
+    philox_block(c, g->key0, g->key1);
+    g->block++;
+    for (int i = 0; i < 4; ++i) u[i] = u_of(c[i]);
+}
+
+/* uniform in (0,1): an odd multiple of 2^-24, exactly representable in FP32 and FP64
+ * (sequential draws of the stream: the raw-stream helper orc_uniforms) */
 static double rng_u(orc_rng* g) {
     if (g->idx == 4) {
         uint32_t c[4] = {g->block, g->id_lo, g->id_hi, 0u};
@@ -81,8 +98,7 @@ static double rng_u(orc_rng* g) {
         g->block++;
         g->idx = 0;
     }
-    uint32_t x = g->buf[g->idx++];
-    return (double)(((x >> 9) << 1) | 1u) * (1.0 / 16777216.0);
+    return u_of(g->buf[g->idx++]);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -376,6 +392,7 @@ typedef struct {
     double weight; /* weight_scatter's packet weight */
     int layer, tflag, reason, bounces;
     orc_rng rng;
+    double tau0; /* optical depth of the next flight (drawn with the preceding event) */
     /* adjoint accumulators */
     int grad;
     double C;                        /* weighted in-grid path tally so far */
@@ -509,7 +526,7 @@ static void tauint2(const orc_scene* S, orc_thread* T, orc_packet* k, int max_bo
     const double eps = 1e-8;
     double ds[SMCRT_MAX_REGIONS], dstmp[SMCRT_MAX_REGIONS];
     v3 oldpos = k->pos;
-    double tau = -log(rng_u(&k->rng)); /* draw: flight optical depth */
+    double tau = k->tau0; /* flight optical depth (drawn at emission or the last interaction) */
     double taurun = 0.0;
     int nr = S->nreg;
     for (;;) {
@@ -573,10 +590,12 @@ static void tauint2(const orc_scene* S, orc_thread* T, orc_packet* k, int max_bo
             /* reflect_refract (src/surfaces.f90:14-40): draw the branch number */
             double Ri = fresnel(k->dir, N, n1, n2);
             int rflag = 0;
-            if (rng_u(&k->rng) <= Ri) { k->dir = reflect(k->dir, N); rflag = 1; }
+            double u[4];
+            rng_block(&k->rng, u); /* the Fresnel event's block */
+            if (u[0] <= Ri) { k->dir = reflect(k->dir, N); rflag = 1; }
             else k->dir = refract(k->dir, N, n1 / n2);
             k->weight *= Ri; /* either branch (:167) */
-            tau = -log(rng_u(&k->rng)); /* draw: resampled optical depth (:168) */
+            tau = -log(u[1]); /* resampled optical depth (:168) */
             taurun = 0.0;
             if (!rflag) {
                 k->layer = new_layer;
@@ -597,17 +616,17 @@ static void tauint2(const orc_scene* S, orc_thread* T, orc_packet* k, int max_bo
         tag_end(k, END_ESCAPE);
 }
 
-/* HG / isotropic scatter (src/photon.f90:713-779) */
-static void scatter(orc_packet* k, double hgg, double g2) {
+/* HG / isotropic scatter (src/photon.f90:713-779); uc, uphi: the cos theta and azimuth draws */
+static void scatter(orc_packet* k, double hgg, double g2, double uc, double uphi) {
     double cost;
     if (hgg == 0.0) {
-        cost = 2.0 * rng_u(&k->rng) - 1.0; /* draw: cos theta */
+        cost = 2.0 * uc - 1.0;
     } else {
-        double temp = (1.0 - g2) / (1.0 - hgg + 2.0 * hgg * rng_u(&k->rng)); /* draw */
+        double temp = (1.0 - g2) / (1.0 - hgg + 2.0 * hgg * uc);
         cost = (1.0 + g2 - temp * temp) / (2.0 * hgg);
     }
     double sint = sqrt(1.0 - cost * cost);
-    double phi = ORC_TWOPI * rng_u(&k->rng); /* draw: azimuth */
+    double phi = ORC_TWOPI * uphi;
     double cosp = cos(phi);
     double sinp = (phi < ORC_PI) ? sqrt(1.0 - cosp * cosp) : -sqrt(1.0 - cosp * cosp);
     v3 u = k->dir, n;
@@ -622,16 +641,19 @@ static void scatter(orc_packet* k, double hgg, double g2) {
     k->dir = n;
 }
 
-/* emitters (src/photon.f90) */
+/* emitters (src/photon.f90); the emission block also gives the first flight's optical depth */
 static void emit(const orc_scene* S, orc_packet* k) {
     const smcrt_source* src = &S->d->source;
+    double u[4];
+    rng_block(&k->rng, u);
+    k->tau0 = -log(u[2]);
     switch (src->kind) {
     case SMCRT_SRC_PENCIL: /* :464-505 */
         k->pos = mk(src->pos[0], src->pos[1], src->pos[2]);
         k->dir = mk(src->dir[0], src->dir[1], src->dir[2]);
         break;
     case SMCRT_SRC_UNIFORM: { /* :397-461 */
-        double rx = rng_u(&k->rng), ry = rng_u(&k->rng); /* draws: the two surface coordinates */
+        double rx = u[0], ry = u[1]; /* the two surface coordinates */
         k->pos = mk(src->p1[0] + rx * src->p2[0] + ry * src->p3[0],
                     src->p1[1] + rx * src->p2[1] + ry * src->p3[1],
                     src->p1[2] + rx * src->p2[2] + ry * src->p3[2]);
@@ -639,16 +661,16 @@ static void emit(const orc_scene* S, orc_packet* k) {
         break;
     }
     case SMCRT_SRC_CIRCULAR: { /* :214-287 */
-        double r = src->radius * sqrt(rng_u(&k->rng)); /* draw: radius */
-        double th = rng_u(&k->rng) * ORC_TWOPI;         /* draw: angle */
+        double r = src->radius * sqrt(u[0]); /* radius */
+        double th = u[1] * ORC_TWOPI;         /* angle */
         v3 p = S->circ_swap ? mk(r * cos(th), r * sin(th), 0.0) : mk(0.0, r * cos(th), r * sin(th));
         k->pos = xform(S->circ_t, p);
         k->dir = mk(src->dir[0], src->dir[1], src->dir[2]);
         break;
     }
     case SMCRT_SRC_POINT: { /* :290-338 */
-        double phi = rng_u(&k->rng) * ORC_TWOPI;  /* draw: azimuth */
-        double cost = 2.0 * rng_u(&k->rng) - 1.0; /* draw: cos theta */
+        double phi = u[0] * ORC_TWOPI;  /* azimuth */
+        double cost = 2.0 * u[1] - 1.0; /* cos theta */
         double sint = sqrt(1.0 - cost * cost);
         k->pos = mk(src->pos[0], src->pos[1], src->pos[2]);
         k->dir = mk(sint * cos(phi), sint * sin(phi), cost);
@@ -858,6 +880,8 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
             int nsc = 0;
             while (!k.tflag && wmode) {
                 /* weight_scatter (src/kernelsMod.f90:83-111) */
+                double u[4];
+                rng_block(&k.rng, u); /* the interaction's block */
                 double wa = k.weight * (1.0 - S.albedo[k.layer]);
                 k.weight -= wa;
                 int ci, cj, ck;
@@ -869,24 +893,28 @@ int orc_run(const smcrt_scene_desc* d, const orc_opts* o, orc_out* out) {
                     out->jmean[idx] += wa;
                 }
                 wdep += wa;
-                scatter(&k, S.hgg[k.layer], S.g2[k.layer]);
+                scatter(&k, S.hgg[k.layer], S.g2[k.layer], u[1], u[2]);
                 nsc++;
                 a_sc++;
                 trace_put(tr, o->trace_events, &k.tc, k.pos, 2);
                 if (k.weight < 0.01) { /* THRESHOLD, CHANCE (src/constants.f90:28-30) */
-                    if (rng_u(&k.rng) < 0.1) k.weight /= 0.1; /* draw: roulette */
+                    if (u[0] < 0.1) k.weight /= 0.1; /* roulette */
                     else { tag_end(&k, END_ABSORB); break; }
                 }
+                k.tau0 = -log(u[3]);
                 tauint2(&S, &T, &k, max_b);
             }
             while (!k.tflag) {
-                if (rng_u(&k.rng) < S.albedo[k.layer]) { /* draw: albedo test */
+                double u[4];
+                rng_block(&k.rng, u); /* the flight end's block */
+                if (u[0] < S.albedo[k.layer]) { /* albedo test */
                     if (k.grad) {
                         int m = S.mat[k.layer];
                         k.K[m] += 1.0;
                         k.B[m] += k.C;
                     }
-                    scatter(&k, S.hgg[k.layer], S.g2[k.layer]);
+                    scatter(&k, S.hgg[k.layer], S.g2[k.layer], u[1], u[2]);
+                    k.tau0 = -log(u[3]);
                     nsc++;
                     a_sc++;
                     trace_put(tr, o->trace_events, &k.tc, k.pos, 2);
@@ -1015,7 +1043,9 @@ void orc_scatter(int64_t n, const double* dir_in, double g, uint64_t seed, uint6
         memset(&k, 0, sizeof k);
         rng_init(&k.rng, seed, id0 + (uint64_t)i);
         k.dir = mk(dir_in[3 * i], dir_in[3 * i + 1], dir_in[3 * i + 2]);
-        scatter(&k, g, g * g);
+        double u[4];
+        rng_block(&k.rng, u);
+        scatter(&k, g, g * g, u[1], u[2]);
         dir_out[3 * i] = k.dir.x; dir_out[3 * i + 1] = k.dir.y; dir_out[3 * i + 2] = k.dir.z;
     }
 }

@@ -160,7 +160,8 @@ SMCRT_DEV void move(Lane& L, float d) {
     move_axis(L.pz, L.rz, d * L.dz);
 }
 
-SMCRT_DEV float rnd(Lane& L, const KParams& P) { return rng_u(L.rng, P.seed_lo, P.seed_hi); }
+// the packet's next event block of four uniforms
+SMCRT_DEV void draw_block(Lane& L, const KParams& P, float u[4]) { rng_block(L.rng, L.pid, P.seed_lo, P.seed_hi, u); }
 
 SMCRT_DEV uint64_t wave_ballot(bool b) { return __ballot(b); }
 
@@ -172,11 +173,15 @@ SMCRT_DEV void trace_event(CKCold* C, Lane& L, int kind) {
     }
 }
 
-// emitters (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338
+// emitters (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338.
+// The emission block: u0, u1 the emitter's draws, u2 the first flight's optical depth.
 SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
+    float u[4];
+    draw_block(L, P, u);
+    L.tau = f_neglog(u[2]);
     switch (C->src_kind) {
     case 2: {  // uniform: draws rx, ry
-        float rx = rnd(L, P), ry = rnd(L, P);
+        float rx = u[0], ry = u[1];
         L.px = C->p1[0] + rx * C->p2[0] + ry * C->p3[0];
         L.py = C->p1[1] + rx * C->p2[1] + ry * C->p3[1];
         L.pz = C->p1[2] + rx * C->p2[2] + ry * C->p3[2];
@@ -184,8 +189,8 @@ SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
         break;
     }
     case 3: {  // circular: draws radius, angle; disc rotated onto dir, centred at -position
-        float r = C->radius * f_sqrt(rnd(L, P));
-        float th = rnd(L, P);
+        float r = C->radius * f_sqrt(u[0]);
+        float th = u[1];
         float c = f_cos_rev(th), s = f_sin_rev(th);
         float qx, qy, qz;
         if (C->circ_swap) { qx = r * c; qy = r * s; qz = 0.f; }
@@ -198,8 +203,8 @@ SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
         break;
     }
     case 4: {  // point: draws azimuth, cos theta
-        float ph = rnd(L, P);
-        float ct = 2.f * rnd(L, P) - 1.f;
+        float ph = u[0];
+        float ct = 2.f * u[1] - 1.f;
         float stt = f_sqrt(fmaxf(1.f - ct * ct, 0.f));
         L.px = C->spos[0]; L.py = C->spos[1]; L.pz = C->spos[2];
         L.dx = stt * f_cos_rev(ph); L.dy = stt * f_sin_rev(ph); L.dz = ct;
@@ -213,19 +218,18 @@ SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
     }
 }
 
-// HG / isotropic scatter (src/photon.f90:713-779).  Draws cos theta, then azimuth.  The
-// near-axis branch keeps the reference's Fortran `sign(cost, nzp)` = |cos theta|*sign(nzp).
-SMCRT_DEV void scatter(const KParams& P, Lane& L, float hgg, float g2) {
+// HG / isotropic scatter (src/photon.f90:713-779); uc, ph: the cos theta and azimuth draws.
+// The near-axis branch keeps the reference's Fortran `sign(cost, nzp)` = |cos theta|*sign(nzp).
+SMCRT_DEV void scatter(Lane& L, float hgg, float g2, float uc, float ph) {
     float cost;
     if (hgg == 0.f) {
-        cost = 2.f * rnd(L, P) - 1.f;
+        cost = 2.f * uc - 1.f;
     } else {
-        float temp = f_div(1.f - g2, 1.f - hgg + 2.f * hgg * rnd(L, P));
+        float temp = f_div(1.f - g2, 1.f - hgg + 2.f * hgg * uc);
         cost = f_div(1.f + g2 - temp * temp, 2.f * hgg);
         cost = fminf(fmaxf(cost, -1.f), 1.f);
     }
     float sint = f_sqrt(fmaxf(1.f - cost * cost, 0.f));
-    float ph = rnd(L, P);
     float cosp = f_cos_rev(ph);
     float sinp = f_sqrt(fmaxf(1.f - cosp * cosp, 0.f));
     if (ph >= 0.5f) sinp = -sinp;  // phi >= pi
@@ -664,7 +668,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
 
         // ---- emission ----
         if (L.st == ST_EMIT) {
-            rng_init(L.rng, L.pid);
+            rng_init(L.rng);
             emit(P, C, L);
             L.rx = 0.f; L.ry = 0.f; L.rz = 0.f;
             L.bounces = 0;
@@ -707,7 +711,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
 
         if (sweeping && L.st == ST_EMIT) {
             L.layer = best;  // nearest containing surface (src/kernelsMod.f90:275-280)
-            L.tau = f_neglog(rnd(L, P));
+            // (the first flight's optical depth was drawn with the emission block)
             L.taurun = 0.f;
             L.iface = false;
             L.st = ST_START;
@@ -813,7 +817,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 const int nl = fn;
                 const float n1 = mt[L.layer].n, n2 = mt[nl].n;
                 float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);
-                bool rflag = rnd(L, P) <= R;  // draw: reflect or refract
+                float u[4];
+                draw_block(L, P, u);  // the Fresnel event's block
+                bool rflag = u[0] <= R;  // reflect or refract
                 L.weight *= R;                // on either branch (src/inttau2.f90:167)
                 if (rflag) {
                     float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
@@ -827,7 +833,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     float k = eta * c1 - c2;
                     L.dx = eta * L.dx + k * nx; L.dy = eta * L.dy + k * ny; L.dz = eta * L.dz + k * nz;
                 }
-                L.tau = f_neglog(rnd(L, P));  // draw: resampled optical depth (:168)
+                L.tau = f_neglog(u[1]);  // resampled optical depth (:168)
                 L.taurun = 0.f;
                 L.iface = true;
                 if (!rflag) {
@@ -966,6 +972,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 }
             }
             move(L, d);
+            float u[4];  // the flight end's block (one site for both estimators)
+            if (W.inside && W.flight_over) draw_block(L, P, u);
             if (!W.inside) {
                 // left the grid (src/inttau2.f90:256-259,282-285): the outcome is fixed.  With
                 // detectors the packet moves on, untallied, to the end of its flight or the next
@@ -996,15 +1004,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     }
                     L.nsc++;
                     out[O_SCATTERS]++;
-                    scatter(P, L, m.hgg, m.g2);
+                    scatter(L, m.hgg, m.g2, u[1], u[2]);
                     trace_event(C, L, 2);
                     bool alive = true;
                     if (L.weight < 0.01f) {
-                        if (rnd(L, P) < 0.1f) L.weight *= 10.f;  // draw: roulette
+                        if (u[0] < 0.1f) L.weight *= 10.f;  // roulette
                         else { end_kind = END_ABSORB; alive = false; }
                     }
                     if (alive) {
-                        L.tau = f_neglog(rnd(L, P));  // draw: next flight's optical depth
+                        L.tau = f_neglog(u[3]);  // next flight's optical depth
                         L.taurun = 0.f;
                         L.iface = false;
                         L.st = ST_START;
@@ -1013,7 +1021,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             } else if (W.flight_over) {
                 // albedo test (src/kernelsMod.f90:287-297)
                 const DevMat m = mt[L.layer];
-                if (rnd(L, P) < m.albedo) {
+                if (u[0] < m.albedo) {
                     if (ADJ && m.mat >= 0) {
                         adj_sc(A, 2, m.mat, P.nmat) += 1.f;
                         adj_sc(A, 3, m.mat, P.nmat) += adjC;
@@ -1026,12 +1034,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                         atomicAdd(mo + 12, (double)L.px * L.px); atomicAdd(mo + 13, (double)L.py * L.py);
                         atomicAdd(mo + 14, (double)L.pz * L.pz);
                     }
-                    scatter(P, L, m.hgg, m.g2);
+                    scatter(L, m.hgg, m.g2, u[1], u[2]);
                     trace_event(C, L, 2);
                     if ((C->flags & 2u) && L.nsc >= 5) {
                         end_kind = END_ABSORB;
                     } else {
-                        L.tau = f_neglog(rnd(L, P));  // draw: next flight's optical depth
+                        L.tau = f_neglog(u[3]);  // next flight's optical depth
                         L.taurun = 0.f;
                         L.iface = false;
                         L.st = ST_START;

@@ -462,11 +462,6 @@ SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, 
 // ------------------------------------------------------------------------------------------
 // Philox-4x32-10 (Salmon et al., SC'11).  Counter (block, id_lo, id_hi, 0), key (seed_lo, seed_hi).
 // ------------------------------------------------------------------------------------------
-struct Rng {
-    uint32_t id_lo, id_hi, block, left;
-    uint32_t b0, b1, b2, b3;
-};
-
 SMCRT_DEV void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -483,24 +478,27 @@ SMCRT_DEV void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, 
     }
 }
 
-SMCRT_DEV void rng_init(Rng& g, uint64_t id) {
-    g.id_lo = (uint32_t)id;
-    g.id_hi = (uint32_t)(id >> 32);
-    g.block = 0;
-    g.left = 0;
-}
-
 // uniform in (0,1): odd multiple of 2^-24, bit-identical to the oracle's draw
-SMCRT_DEV float rng_u(Rng& g, uint32_t k0, uint32_t k1) {
-    if (g.left == 0) {
-        uint32_t c0 = g.block, c1 = g.id_lo, c2 = g.id_hi, c3 = 0u;
-        philox10(c0, c1, c2, c3, k0, k1);
-        g.b0 = c0; g.b1 = c1; g.b2 = c2; g.b3 = c3;
-        g.block++;
-        g.left = 4;
-    }
-    uint32_t x = g.b0;
-    g.b0 = g.b1; g.b1 = g.b2; g.b2 = g.b3;
-    g.left--;
-    return (float)(((x >> 9) << 1) | 1u) * (1.0f / 16777216.0f);
+SMCRT_DEV float u_of(uint32_t x) { return (float)(((x >> 9) << 1) | 1u) * (1.0f / 16777216.0f); }
+
+// A packet's stream: one Philox block per event, counter (block, id_lo, id_hi, 0), key (seed).
+// The events and the use of their four uniforms (the oracle's mapping): emission u0, u1 for the
+// emitter, u2 the first flight's optical depth; flight end u0 albedo test (weight estimator:
+// roulette), u1 cos theta, u2 azimuth, u3 the next optical depth; Fresnel u0 reflect or
+// refract, u1 the resampled optical depth.  One Philox call per event keeps one inlined copy
+// per event site, and no buffered draws in registers.
+struct Rng {
+    uint32_t block;
+};
+
+SMCRT_DEV void rng_init(Rng& g) { g.block = 0u; }
+
+SMCRT_DEV void rng_block(Rng& g, uint64_t id, uint32_t k0, uint32_t k1, float u[4]) {
+    uint32_t c0 = g.block, c1 = (uint32_t)id, c2 = (uint32_t)(id >> 32), c3 = 0u;
+    philox10(c0, c1, c2, c3, k0, k1);
+    g.block++;
+    u[0] = u_of(c0);
+    u[1] = u_of(c1);
+    u[2] = u_of(c2);
+    u[3] = u_of(c3);
 }

@@ -13,7 +13,7 @@
  *     (src/random_mod.f90:83-90), whose bit stream is compiler-specific.  Here every packet draws
  *     from its own Philox-4x32-10 stream keyed by (seed, global packet id), the same stream the
  *     GPU kernel uses, so the two can be compared packet by packet for the first events
- *     (common random numbers).  Draw order is fixed and documented at each draw.
+ *     (common random numbers).  Each event takes one Philox block (see rng_block).
  *   - The fluence accumulator is FP64 (the reference adds FP32 `real(dcell,sp)` into an FP32
  *     `jmean`, src/inttau2.f90:272; src/iarray.f90:12).
  *   - `error stop` paths (src/inttau2.f90:156,337-343) increment diagnostic counters and end the

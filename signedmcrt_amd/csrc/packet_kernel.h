@@ -304,6 +304,56 @@ SMCRT_DEV T prim_node(CDevNode* nodes, int i, const V3<T>& p, const float* __res
 
 SMCRT_DEV V3<float> dual_grad(const Dual& d) { return V3<float>{d.x, d.y, d.z}; }
 
+// Closed-form gradient of a single sphere or box node at p (the derivative the Dual evaluation
+// of the same expressions gives, branch for branch: |x| has slope +1 at 0, max keeps its first
+// argument on ties, sqrt has slope 0 at 0), pulled back through the node's affine transform.
+// Returns false for other kinds (the caller differentiates them with Dual numbers).
+template <uint32_t KM>
+SMCRT_DEV bool fast_prim_gradient(CDevNode* nodes, int i, const V3<float>& p, V3<float>& g) {
+    CDevNode* n = nodes + __builtin_amdgcn_readfirstlane(i);
+    const int kind = n->kind;
+    const bool sphere = KM_HAS(KM, KM_PRIM(SMCRT_SPHERE)) && kind == SMCRT_SPHERE;
+    const bool box = KM_HAS(KM, KM_PRIM(SMCRT_BOX)) && kind == SMCRT_BOX;
+    if (n->op != SMCRT_OP_PRIM || !(sphere || box)) return false;
+    float tt[12];
+    V3<float> q = p;
+    const bool ident = n->ident != 0;
+    if (!ident) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
+        q = xform<float>(tt, p);
+    }
+    V3<float> gq;
+    if (sphere) {
+        const float s = f_sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+        const float h = s > 0.f ? f_rcp(s) : 0.f;
+        gq = V3<float>{q.x * h, q.y * h, q.z * h};
+    } else {
+        const float sx = q.x < 0.f ? -1.f : 1.f, sy = q.y < 0.f ? -1.f : 1.f, sz = q.z < 0.f ? -1.f : 1.f;
+        const float dx = fabsf(q.x) - n->p[0], dy = fabsf(q.y) - n->p[1], dz = fabsf(q.z) - n->p[2];
+        const float mx = dx > 0.f ? dx : 0.f, my = dy > 0.f ? dy : 0.f, mz = dz > 0.f ? dz : 0.f;
+        const float s = f_sqrt(mx * mx + my * my + mz * mz);
+        const float h = s > 0.f ? f_rcp(s) : 0.f;
+        gq = V3<float>{sx * mx * h, sy * my * h, sz * mz * h};
+        // min(max(dx, max(dy, dz)), 0): the largest component when negative
+        const bool zy = dz > dy;
+        const float inner = zy ? dz : dy;
+        const bool ix = inner > dx;
+        if ((ix ? inner : dx) < 0.f) {
+            if (!ix) gq.x += sx;
+            else if (zy) gq.z += sz;
+            else gq.y += sy;
+        }
+    }
+    if (ident) {
+        g = gq;
+    } else {  // q_j = sum_i p_i t[3i + j] + t[9 + j]
+        g = V3<float>{gq.x * tt[0] + gq.y * tt[1] + gq.z * tt[2], gq.x * tt[3] + gq.y * tt[4] + gq.z * tt[5],
+                      gq.x * tt[6] + gq.y * tt[7] + gq.z * tt[8]};
+    }
+    return true;
+}
+
 // Exact gradient of region r at p.  A flat fold of primitives by union or SmoothUnion (a `model`
 // of primitives, DevRegion.flat) is differentiated lazily, forward-mode through the fold:
 // running value a, gradient g; each child c_k enters as a <- op(a, c_k), g <- alpha g + beta
@@ -319,6 +369,10 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
                                     const float* __restrict__ accel) {
     const V3<Dual> pd{Dual{p.x, 1.f, 0.f, 0.f}, Dual{p.y, 0.f, 1.f, 0.f}, Dual{p.z, 0.f, 0.f, 1.f}};
     // flat folds exist only in programs with a union or SmoothUnion
+    if (count == 1) {
+        V3<float> g;
+        if (fast_prim_gradient<KM>(nodes, first, p, g)) return g;
+    }
     if constexpr (!KM_HAS(KM, KM_CSG(1) | KM_CSG(2))) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
     if (!flat) return dual_grad(region_eval<Dual, KM>(nodes, first, count, pd, accel));
     const int nch = (count + 1) / 2;

@@ -6,7 +6,7 @@
 // (src/inttau2.f90:15-210):
 //
 //   EMIT     sweep at the emission point: layer = nearest containing surface
-//            (src/kernelsMod.f90:275-280), draw tau, then as START
+//            (src/kernelsMod.f90:275-280) (tau was drawn with the emission), then as START
 //   START    flight start (src/inttau2.f90:52-61): kill if within eps of a surface, else march
 //   MARCHED  after a step (:85-97): escape if outside every region, march again while d > eps,
 //            otherwise cross: step min|d| + 2 eps along dir (:104-116)
@@ -18,12 +18,15 @@
 // path-length estimator of `update_grids` (:213-292).  A walk advances at most `dda_cap` chords
 // per iteration and resumes in the next one, so a few long walks do not stall the wavefront.
 // Consecutive chords in the same voxel (a step ending and the next one starting in it) are
-// merged before the FP64 atomic is issued.  When the flight's optical depth runs out, the
-// albedo test absorbs or scatters (HG, src/photon.f90:713-779) and a new flight starts.
+// merged; a block keeps the hottest voxel box in an LDS tile, and the other chords go through a
+// per-wavefront LDS queue to FP64 atomics into its XCD's partial grid.  When the flight's optical
+// depth runs out, the albedo test absorbs or scatters (HG, src/photon.f90:713-779) and a new
+// flight starts.
 // A lane whose packet ends takes the next packet id from its wavefront's chunk at the top of
 // the next iteration (regeneration); a wavefront claims a chunk of ids from a global counter
-// with one atomic.  Packet j draws from Philox stream (seed, first_packet + j), so results do
-// not depend on grid size, block size, chunking or the walk cap.
+// with one atomic.  Packet j draws from Philox stream (seed, first_packet + j), one block per
+// event, so results do not depend on grid size, block size, chunking, the walk cap or the
+// number of resident waves.
 #pragma once
 #include "smcrt_device.h"
 

@@ -594,9 +594,20 @@ SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
     return true;
 }
 
+// How the sweep evaluates the scene's regions.  The library's kernels interpret the SDF program
+// (region_eval); a kernel compiled at run time for one scene passes a policy whose region<KM>(r,
+// p, accel) is that region's program as straight-line code (constants as immediates, no node
+// loads or opcode dispatch, independent regions interleaved by the compiler).
+struct InterpretedSweep {
+    static constexpr bool kCompiled = false;
+    static constexpr int kRegions = 0;
+    template <uint32_t KM>
+    static SMCRT_DEV float region(int, const V3<float>&, const float* __restrict__) { return 0.f; }
+};
+
 // WAVES: resident waves per SIMD the registers are allocated for (2: at most 256 VGPRs, 3: at most
-// 168, with some spilled in the specialised forward kernels)
-template <bool ADJ, uint32_t KM, int WAVES = 2>
+// 168).  SW: the sweep policy.
+template <bool ADJ, uint32_t KM, int WAVES = 2, class SW = InterpretedSweep>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     CDevNode* const nodes = (CDevNode*)P.nodes;
@@ -751,8 +762,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         bool runaway = false;
         if (sweeping) {
             V3<float> p{L.px, L.py, L.pz};
-            for (int r = 0; r < P.nreg; ++r) {
-                float d = region_eval<float, KM>(nodes, regions[r].first, regions[r].count, p, P.accel);
+            auto visit = [&](int r, float d) {
                 dmin = fminf(dmin, fabsf(d));
                 smin = fminf(smin, d);
                 if (r == 0) d0 = d;
@@ -760,6 +770,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 if (r == L.layer) dlay = d;
                 float c = d > 0.f ? 999.f : -d;  // positive distances count as -999
                 if (c < bestv) { bestv = c; best = r; }
+            };
+            if constexpr (SW::kCompiled) {
+                // the scene's program compiled into the kernel: straight-line code per region
+#pragma unroll
+                for (int r = 0; r < SW::kRegions; ++r) visit(r, SW::template region<KM>(r, p, P.accel));
+            } else {
+                for (int r = 0; r < P.nreg; ++r)
+                    visit(r, region_eval<float, KM>(nodes, regions[r].first, regions[r].count, p, P.accel));
             }
             if (fn < 0) { fn = 0; dfirst = d0; }  // minloc of an all-zero array is index 1
             runaway = ++L.iters > (1u << 24);     // runaway guard: never expected

@@ -2,18 +2,23 @@
 // packet-loop launches (single scene, adjoint, batched + HIP graph), SDF evaluation and the
 // host-side output helpers.
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/smcrt.h"
 #include "packet_kernel.h"
+#include "jit_sources.inc"  // the device headers as strings (generated at build time)
 
 using namespace smcrt;
 
@@ -151,6 +156,15 @@ struct DevBuf {
     template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Packet kernels compiled at run time for one scene's SDF program (see jit_build).
+struct JitKernels {
+    bool tried = false, ok = false, cached = false;
+    hipModule_t mod = nullptr;
+    hipFunction_t fwd[2] = {nullptr, nullptr}, adj[2] = {nullptr, nullptr};  // [waves - 2]
+    std::string error;
+    double seconds = 0.0;
+};
+
 struct smcrt_scene {
     std::vector<DevNode> nodes;
     std::vector<DevRegion> regions;
@@ -166,6 +180,8 @@ struct smcrt_scene {
     std::vector<float> accel;
     // primitive, operator and modifier kinds the program uses (KM_* bits of smcrt_device.h)
     uint32_t kind_mask = 0;
+    JitKernels jit;
+    std::string kernel_name;  // smcrt_scene_kernel's answer
     // detectors (prepared for the device) and the offset of each one's bins
     std::vector<smcrt_detector> dets;
     std::vector<DevDet> ddets;
@@ -656,6 +672,7 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
         if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->jit.mod) (void)hipModuleUnload(s->jit.mod);
     }
     delete s;
     return SMCRT_OK;
@@ -793,6 +810,7 @@ static void fill_common(const smcrt_scene* s, KParams& P, KCold& Cc) {
 
 struct LaunchShape {
     int block, blocks, chunk, walk_cap, refill_min, waves;
+    bool compiled;  // the scene's run-time compiled kernels (JitKernels) instead of the library's
     size_t lds;
 };
 
@@ -848,9 +866,291 @@ static int launch_waves(const smcrt_scene* s, const smcrt_run_opts* o, bool adj)
     return (w == 3 && (adj ? v.adj3 : v.fwd3)) ? 3 : 2;
 }
 
+// ---- packet kernels compiled at run time for one scene -------------------------------------
+// The library's kernels interpret the SDF program; for large launches the scene's program is
+// turned into straight-line device code (one `case` per region: every node's parameters as
+// exact hexadecimal float constants, the interpreter's value and position stacks as named
+// temporaries), compiled with hipRTC together with the packet kernel (the device headers are
+// embedded in the library), and launched from the loaded module.  Same device functions, same
+// arithmetic: only the node loads and the opcode dispatch disappear.  The code object is cached
+// on disk by a hash of its source.  Any failure leaves the library's kernels in use.
+
+static void jit_float(std::string& o, float v) {
+    char b[64];
+    if (std::isnan(v)) snprintf(b, sizeof b, "__builtin_nanf(\"\")");
+    else if (std::isinf(v)) snprintf(b, sizeof b, v > 0.f ? "__builtin_inff()" : "(-__builtin_inff())");
+    else snprintf(b, sizeof b, "%af", (double)v);
+    o += b;
+}
+
+static void jit_array(std::string& o, const std::string& name, const float* v, int n) {
+    o += "            const float " + name + "[" + std::to_string(n) + "] = {";
+    for (int i = 0; i < n; ++i) {
+        if (i) o += ", ";
+        jit_float(o, v[i]);
+    }
+    o += "};\n";
+}
+
+// Instantiation name expressions: [adj][waves - 2]
+static std::string jit_name(uint32_t km, bool adj, int waves) {
+    return std::string("smcrt::packet_kernel<") + (adj ? "true" : "false") + ", " + std::to_string(km) + "u, " +
+           std::to_string(waves) + ", smcrt::SceneSweep>";
+}
+
+static std::string jit_source(const smcrt_scene* s, uint32_t km) {
+    std::string o = "#include \"smcrt.h\"\n#include \"packet_kernel.h\"\nnamespace smcrt {\n"
+                    "struct SceneSweep {\n"
+                    "    static constexpr bool kCompiled = true;\n"
+                    "    static constexpr int kRegions = " + std::to_string(s->regions.size()) + ";\n"
+                    "    template <uint32_t KM>\n"
+                    "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel) {\n"
+                    "        switch (r) {\n";
+    for (size_t r = 0; r < s->regions.size(); ++r) {
+        const DevRegion& R = s->regions[r];
+        o += "        case " + std::to_string(r) + ": {\n";
+        std::vector<std::string> vs, ps{"p"};
+        for (int i = 0; i < R.count; ++i) {
+            const DevNode& n = s->nodes[R.first + i];
+            const std::string id = std::to_string(i), kind = std::to_string(n.kind);
+            if (n.op == SMCRT_OP_PRIM) {
+                jit_array(o, "P" + id, n.p, 8);
+                jit_array(o, "Q" + id, n.q, 8);
+                std::string q = ps.back();
+                if (!n.ident) {
+                    jit_array(o, "T" + id, n.t, 12);
+                    o += "            const V3<float> x" + id + " = xform<float>(T" + id + ", " + q + ");\n";
+                    q = "x" + id;
+                }
+                o += "            const float v" + id + " = prim_eval<float, KM>(" + kind + ", P" + id + ", Q" + id + ", " +
+                     q + ", accel);\n";
+                vs.push_back("v" + id);
+            } else if (n.op == SMCRT_OP_CSG) {
+                jit_array(o, "Q" + id, n.q, 8);
+                const std::string b = vs.back();
+                vs.pop_back();
+                const std::string a = vs.back();
+                vs.pop_back();
+                o += "            const float v" + id + " = csg_eval<float, KM>(" + kind + ", " + a + ", " + b + ", ";
+                jit_float(o, n.k);
+                o += ", Q" + id + ");\n";
+                vs.push_back("v" + id);
+            } else if (n.op == SMCRT_OP_MOD_BEGIN) {
+                jit_array(o, "P" + id, n.p, 8);
+                o += "            const V3<float> x" + id + " = mod_pos<float, KM>(" + kind + ", P" + id + ", " + ps.back() +
+                     ");\n";
+                ps.push_back("x" + id);
+            } else {  // MOD_END: the modifier's post-step at the enclosing position
+                jit_array(o, "P" + id, n.p, 8);
+                ps.pop_back();
+                const std::string v = vs.back();
+                vs.pop_back();
+                o += "            const float v" + id + " = mod_post<float, KM>(" + kind + ", P" + id + ", " + ps.back() +
+                     ", " + v + ");\n";
+                vs.push_back("v" + id);
+            }
+        }
+        o += "            return " + (vs.empty() ? std::string("0.f") : vs.back()) + ";\n        }\n";
+    }
+    o += "        }\n        return 0.f;\n    }\n};\n";
+    for (int adj = 0; adj < 2; ++adj)
+        for (int w = 2; w <= 3; ++w) o += "template __global__ void " + jit_name(km, adj, w).substr(7) + "(KParams);\n";
+    o += "}  // namespace smcrt\n";
+    return o;
+}
+
+static uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+static std::string jit_cache_dir() {
+    const char* d = getenv("SMCRT_CACHE_DIR");
+    std::string dir;
+    if (d && *d) dir = d;
+    else {
+        const char* home = getenv("HOME");
+        dir = std::string(home && *home ? home : "/tmp") + "/.cache/smcrt";
+    }
+    dir += "/jit";
+    // mkdir -p
+    for (size_t i = 1; i <= dir.size(); ++i)
+        if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+    return dir;
+}
+
+// cache file: u32 count, then per name (u32 length, bytes), then u64 code size and the code
+static bool jit_cache_read(const std::string& path, std::vector<std::string>& names, std::vector<char>& code) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    bool ok = true;
+    uint32_t n = 0;
+    ok = ok && fread(&n, 4, 1, f) == 1 && n == 4;
+    names.assign(ok ? n : 0, std::string());
+    for (uint32_t i = 0; ok && i < n; ++i) {
+        uint32_t len = 0;
+        ok = fread(&len, 4, 1, f) == 1 && len < 4096;
+        if (ok) {
+            names[i].resize(len);
+            ok = fread(&names[i][0], 1, len, f) == len;
+        }
+    }
+    uint64_t cs = 0;
+    ok = ok && fread(&cs, 8, 1, f) == 1 && cs > 0 && cs < (1ull << 30);
+    if (ok) {
+        code.resize(cs);
+        ok = fread(code.data(), 1, cs, f) == cs;
+    }
+    fclose(f);
+    return ok;
+}
+
+static void jit_cache_write(const std::string& path, const std::vector<std::string>& names, const std::vector<char>& code) {
+    const std::string tmp = path + ".tmp" + std::to_string((unsigned long)getpid());
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    bool ok = true;
+    const uint32_t n = (uint32_t)names.size();
+    ok = ok && fwrite(&n, 4, 1, f) == 1;
+    for (const std::string& x : names) {
+        const uint32_t len = (uint32_t)x.size();
+        ok = ok && fwrite(&len, 4, 1, f) == 1 && fwrite(x.data(), 1, len, f) == len;
+    }
+    const uint64_t cs = code.size();
+    ok = ok && fwrite(&cs, 8, 1, f) == 1 && fwrite(code.data(), 1, cs, f) == cs;
+    ok = (fclose(f) == 0) && ok;
+    if (ok) (void)rename(tmp.c_str(), path.c_str());
+    else (void)remove(tmp.c_str());
+}
+
+// Compile (or load from the cache) the scene's kernels; true when they are usable.
+static bool jit_build(smcrt_scene* s) {
+    JitKernels& J = s->jit;
+    if (J.tried) return J.ok;
+    J.tried = true;
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t km = s->kind_mask;
+    const std::string src = jit_source(s, km);
+    static const char* const opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-munsafe-fp-atomics",
+                                       "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const int nopts = (int)(sizeof opts / sizeof opts[0]);
+    uint64_t h = fnv1a(src.data(), src.size());
+    for (int i = 0; i < kJitHeaders; ++i) h = fnv1a(kJitHeaderSources[i], strlen(kJitHeaderSources[i]), h);
+    for (int i = 0; i < nopts; ++i) h = fnv1a(opts[i], strlen(opts[i]), h);
+    char hex[32];
+    snprintf(hex, sizeof hex, "%016llx", (unsigned long long)h);
+    const std::string path = jit_cache_dir() + "/packet_kernel_" + hex + ".co";
+    std::vector<std::string> names;
+    std::vector<char> code;
+    if (jit_cache_read(path, names, code)) {
+        J.cached = true;
+    } else {
+        // hipRTC has no C library headers: stddef.h / stdint.h / hip_runtime.h come from its own
+        // runtime header, so the headers the sources include are given as stubs
+        std::vector<const char*> hs, hn;
+        for (int i = 0; i < kJitHeaders; ++i) {
+            hs.push_back(kJitHeaderSources[i]);
+            hn.push_back(kJitHeaderNames[i]);
+        }
+        static const char* const stdint_stub =
+            "#pragma once\n"
+            "typedef __hip_internal::int8_t int8_t;\ntypedef __hip_internal::int16_t int16_t;\n"
+            "typedef __hip_internal::int32_t int32_t;\ntypedef __hip_internal::int64_t int64_t;\n"
+            "typedef __hip_internal::uint8_t uint8_t;\ntypedef __hip_internal::uint16_t uint16_t;\n"
+            "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n";
+        hs.push_back("#pragma once\n");
+        hn.push_back("stddef.h");
+        hs.push_back(stdint_stub);
+        hn.push_back("stdint.h");
+        hs.push_back("#pragma once\n");
+        hn.push_back("hip/hip_runtime.h");
+        hiprtcProgram prog = nullptr;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "scene_packet_kernel.hip", (int)hs.size(), hs.data(), hn.data()) !=
+            HIPRTC_SUCCESS) {
+            J.error = "hiprtcCreateProgram failed";
+            return false;
+        }
+        std::vector<std::string> exprs;
+        for (int adj = 0; adj < 2; ++adj)
+            for (int w = 2; w <= 3; ++w) {
+                exprs.push_back(jit_name(km, adj, w));
+                (void)hiprtcAddNameExpression(prog, exprs.back().c_str());
+            }
+        const hiprtcResult rc = hiprtcCompileProgram(prog, nopts, opts);
+        if (rc != HIPRTC_SUCCESS) {
+            size_t ls = 0;
+            (void)hiprtcGetProgramLogSize(prog, &ls);
+            std::string log(ls + 1, '\0');
+            (void)hiprtcGetProgramLog(prog, &log[0]);
+            J.error = std::string("hipRTC: ") + hiprtcGetErrorString(rc) + ": " + log.substr(0, 2000);
+            (void)hiprtcDestroyProgram(&prog);
+            return false;
+        }
+        for (const std::string& e : exprs) {
+            const char* lowered = nullptr;
+            if (hiprtcGetLoweredName(prog, e.c_str(), &lowered) != HIPRTC_SUCCESS || !lowered) {
+                J.error = "hipRTC: no lowered name for " + e;
+                (void)hiprtcDestroyProgram(&prog);
+                return false;
+            }
+            names.push_back(lowered);
+        }
+        size_t cs = 0;
+        if (hiprtcGetCodeSize(prog, &cs) != HIPRTC_SUCCESS || cs == 0) {
+            J.error = "hipRTC: empty code object";
+            (void)hiprtcDestroyProgram(&prog);
+            return false;
+        }
+        code.resize(cs);
+        (void)hiprtcGetCode(prog, code.data());
+        (void)hiprtcDestroyProgram(&prog);
+        jit_cache_write(path, names, code);
+    }
+    if (hipSetDevice(s->device) != hipSuccess || hipModuleLoadData(&J.mod, code.data()) != hipSuccess) {
+        J.error = "hipModuleLoadData failed";
+        J.mod = nullptr;
+        return false;
+    }
+    hipFunction_t* fn[4] = {&J.fwd[0], &J.fwd[1], &J.adj[0], &J.adj[1]};
+    for (int i = 0; i < 4; ++i)
+        if (hipModuleGetFunction(fn[i], J.mod, names[i].c_str()) != hipSuccess) {
+            J.error = "hipModuleGetFunction failed for " + names[i];
+            (void)hipModuleUnload(J.mod);
+            J.mod = nullptr;
+            return false;
+        }
+    J.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    J.ok = true;
+    return true;
+}
+
+// Run-time compiled kernels for this launch?  SMCRT_JIT=0 never, =1 always; by default for
+// launches of at least 2^22 packets (a compile costs a few seconds once per program; the disk
+// cache makes later processes load it directly).  Batches always use the library's kernels.
+static bool jit_wanted(const smcrt_run_opts* o) {
+    const char* e = getenv("SMCRT_JIT");
+    if (e && !strcmp(e, "0")) return false;
+    if (e && !strcmp(e, "1")) return true;
+    return o && o->n_packets >= (1ull << 22);
+}
+
+static hipFunction_t jit_function(const smcrt_scene* s, bool adj, int waves) {
+    return adj ? s->jit.adj[waves - 2] : s->jit.fwd[waves - 2];
+}
+
+static hipError_t kernel_occupancy(const smcrt_scene* s, bool adj, int waves, bool compiled, int block, size_t lds,
+                                   int* active) {
+    if (compiled) return hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(active, jit_function(s, adj, waves), block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(active, packet_kernel_for(s, adj, waves), block, lds);
+}
+
 static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const LaunchShape& sh, hipStream_t stream,
                                        KParams& P) {
     void* args[] = {&P};
+    if (sh.compiled)
+        return hipModuleLaunchKernel(jit_function(s, adj, sh.waves), (unsigned)sh.blocks, 1, 1, (unsigned)sh.block, 1, 1,
+                                     (unsigned)sh.lds, stream, args, nullptr);
     return hipLaunchKernel((const void*)packet_kernel_for(s, adj, sh.waves), dim3(sh.blocks), dim3(sh.block), args,
                            sh.lds, stream);
 }
@@ -858,14 +1158,15 @@ static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const Lau
 // Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
 // material tables of n_sets sets, plus the adjoint's per-lane scores and gradients).
 static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0,
-                              int adj_slots = 0) {
+                              int adj_slots = 0, bool compiled = false) {
     LaunchShape L;
+    L.compiled = compiled;
     L.block = (o && o->block_size > 0) ? o->block_size : 256;
     int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
     L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
-    L.waves = launch_waves(s, o, adj);
+    L.waves = compiled ? ((o && (o->waves == 2)) ? 2 : 3) : launch_waves(s, o, adj);
     const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
@@ -882,8 +1183,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once
         int active = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&active, packet_kernel_for(s, adj, L.waves), L.block,
-                                                                   L.lds);
+        hipError_t e = kernel_occupancy(s, adj, L.waves, compiled, L.block, L.lds, &active);
         if (e == hipSuccess && active > 0) bpc = active;
     }
     L.blocks = s->num_cu * bpc;
@@ -1035,13 +1335,15 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     }
     Cc.flags = o->flags;
     Cc.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
+    // the scene's run-time compiled kernels (built once per scene; on failure the library's)
+    const bool compiled = jit_wanted(o) && jit_build(s);
     int tile_vox = 0;
     if (use_tile) {
-        const int wi = launch_waves(s, o, adj) - 2;
+        const int wi = (compiled ? (o->waves == 2 ? 2 : 3) : launch_waves(s, o, adj)) - 2;
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }
-    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat);
+    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat, compiled);
     P.chunk = sh.chunk;
     P.dda_cap = sh.walk_cap;
     P.refill_min = sh.refill_min;
@@ -1548,5 +1850,9 @@ extern "C" const char* smcrt_scene_kernel(const smcrt_scene* s, uint32_t* scene_
     const KernelVariant& v = kernel_variant(s);
     if (scene_mask) *scene_mask = s->kind_mask;
     if (variant_mask) *variant_mask = v.mask;
-    return v.name;
+    smcrt_scene* m = const_cast<smcrt_scene*>(s);
+    m->kernel_name = v.name;
+    if (s->jit.ok) m->kernel_name += s->jit.cached ? "+compiled(cached)" : "+compiled";
+    else if (s->jit.tried) m->kernel_name += "+compile-failed: " + s->jit.error;
+    return m->kernel_name.c_str();
 }

@@ -1,0 +1,105 @@
+"""Packet kernels compiled at run time for one scene's SDF program (hipRTC; straight-line region
+code instead of the interpreter) against the library's interpreting kernels on the same packets.
+
+The compiled code calls the same device functions on the same float constants, so packets follow
+the same histories; the compiler may fuse multiply-adds differently around constants, so a rare
+history can branch differently: counters agree to 1e-3 relative, the fluence sum to 1e-3."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from signedmcrt_amd import scenes
+from signedmcrt_amd.scene import Scene, cart_grid, source
+from signedmcrt_amd.sdf import box, mono
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("absorbed", "escaped_r", "escaped_t", "killed_surface", "scatters", "ev_crossings")
+
+
+def _close(a, b, rel=1e-3, absolute=20):
+    return abs(a - b) <= max(rel * max(abs(a), abs(b)), absolute)
+
+
+class _Mode:
+    """Engine whose launches run with SMCRT_JIT set to `mode` (read at each launch)."""
+
+    def __init__(self, sc, mode, monkeypatch):
+        from signedmcrt_amd.engine import Engine
+        self.eng, self.mode, self.mp = Engine(sc), mode, monkeypatch
+
+    def __getattr__(self, name):
+        f = getattr(self.eng, name)
+        if not callable(f):
+            return f
+
+        def call(*a, **kw):
+            self.mp.setenv("SMCRT_JIT", self.mode)
+            return f(*a, **kw)
+        return call
+
+
+def _pair(sc, monkeypatch, tmp_path):
+    monkeypatch.setenv("SMCRT_CACHE_DIR", str(tmp_path))
+    return _Mode(sc, "0", monkeypatch), _Mode(sc, "1", monkeypatch)
+
+
+def _zoo_scene():
+    from test_gpu_parity import sdf_zoo
+    arr = sdf_zoo() + [box((4.0, 4.0, 4.0), mono(0.0, 0.0, 0.0, 1.0), 2)]
+    # a point source off every region's surface (the origin lies on several of them)
+    return Scene(arr, cart_grid(32, 32, 32, 2.0, 2.0, 2.0), source("point", (0.05, 0.13, 0.21)))
+
+
+@pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels", "polygon_regions", "zoo"])
+def test_compiled_kernels_match_interpreter(gpu, name, monkeypatch, tmp_path):
+    if name == "zoo":
+        sc = _zoo_scene()
+    else:
+        kw = {"nxyz": 32} if name == "skin_vessels" else ({"n_side": 2, "n_edges": 40, "nxy": 40, "nz": 8}
+                                                          if name == "polygon_regions" else {})
+        sc, _ = scenes.build(name, **kw)
+    interp, comp = _pair(sc, monkeypatch, tmp_path)
+    a = interp.run(300_000, seed=23)
+    b = comp.run(300_000, seed=23)
+    assert "compiled" in comp.kernel_variant()[0], comp.kernel_variant()
+    assert "compiled" not in interp.kernel_variant()[0]
+    for k in KEYS:
+        assert _close(a.stats[k], b.stats[k]), (k, a.stats[k], b.stats[k])
+    assert b.stats["err_unreachable"] == 0 and b.stats["err_nan"] == 0
+    # (the overlapping zoo regions kill most packets at interfaces, on the oracle too)
+    assert b.stats["killed_surface"] < b.stats["launched"] and b.stats["scatters"] > 0
+    assert abs(a.jmean.sum() / b.jmean.sum() - 1) < 1e-3
+    m = a.jmean > 0.05 * a.jmean.max()
+    assert np.median(np.abs(a.jmean[m] / b.jmean[m] - 1)) < 1e-3
+    # 2- and 3-wave compiled builds: the same packets
+    c = comp.run(300_000, seed=23, waves=2)
+    for k in KEYS:
+        assert c.stats[k] == b.stats[k], k
+
+
+def test_compiled_adjoint_matches_interpreter(gpu, monkeypatch, tmp_path):
+    sc, _ = scenes.build("skin_vessels", nxyz=32)
+    interp, comp = _pair(sc, monkeypatch, tmp_path)
+    ga = interp.grad(100_000, seed=29).d_w_dmua
+    gb = comp.grad(100_000, seed=29).d_w_dmua
+    assert "compiled" in comp.kernel_variant()[0]
+    np.testing.assert_allclose(gb, ga, rtol=2e-3, atol=1e-3 * np.abs(ga).max())
+
+
+def test_compiled_kernels_are_cached(gpu, monkeypatch, tmp_path):
+    """A second scene handle with the same program loads the code object from the disk cache."""
+    from signedmcrt_amd.engine import Engine
+    monkeypatch.setenv("SMCRT_CACHE_DIR", str(tmp_path))
+    monkeypatch.setenv("SMCRT_JIT", "1")
+    sc, _ = scenes.build("refracting_sphere", nxyz=32)
+    e1 = Engine(sc)
+    r1 = e1.run(50_000, seed=3, tally=False)
+    assert e1.kernel_variant()[0] == "basic+compiled"
+    assert len(list((tmp_path / "jit").glob("*.co"))) == 1
+    e2 = Engine(sc)
+    r2 = e2.run(50_000, seed=3, tally=False)
+    assert e2.kernel_variant()[0] == "basic+compiled(cached)"
+    for k in KEYS:
+        assert r1.stats[k] == r2.stats[k], k

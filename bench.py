@@ -62,6 +62,15 @@ def per_packet_rel_sd(batches):
     return float(np.median(sd) * np.sqrt(n_b))
 
 
+def kernel_label(eng):
+    """The forward kernel's name as rocprof reports it: the library's variant, or the kernel
+    compiled at run time for the scene's program (instantiated on the scene's exact kinds)."""
+    name, scene_mask, variant_mask = eng.kernel_variant()
+    if "compiled" in name:
+        return "packet_kernel<false, %du, 3, SceneSweep> (%s)" % (scene_mask, name)
+    return "packet_kernel<false, %du> (%s)" % (variant_mask, name)
+
+
 def read_traffic(workload):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     p = os.path.join(ROOT, "profiles", f"pmc_packet_kernel_{workload}.md")
@@ -258,7 +267,7 @@ def main():
                    "launch_shape": tuned["shape"] if tuned else "library default"},
         "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP32_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": "packet_kernel<false, %du> (%s)" % (eng.kernel_variant()[2], eng.kernel_variant()[0]),
+                     "traffic": traffic, "kernel": kernel_label(eng),
                      "kernel_ms": kern_ms, "flops_per_launch": flops_launch},
         "roofline_tally": {"bound": "hbm", "achieved": tally_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": tally_gbs / HBM_PEAK_GBS,

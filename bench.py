@@ -62,13 +62,13 @@ def per_packet_rel_sd(batches):
     return float(np.median(sd) * np.sqrt(n_b))
 
 
-def kernel_label(eng):
+def kernel_label(eng, waves):
     """The forward kernel's name as rocprof reports it: the library's variant, or the kernel
     compiled at run time for the scene's program (instantiated on the scene's exact kinds)."""
     name, scene_mask, variant_mask = eng.kernel_variant()
     if "compiled" in name:
-        return "packet_kernel<false, %du, 3, SceneSweep> (%s)" % (scene_mask, name)
-    return "packet_kernel<false, %du> (%s)" % (variant_mask, name)
+        return "packet_kernel<false, %du, %d, SceneSweep> (%s)" % (scene_mask, waves, name)
+    return "packet_kernel<false, %du, %d> (%s)" % (variant_mask, waves, name)
 
 
 def read_traffic(workload):
@@ -267,7 +267,7 @@ def main():
                    "launch_shape": tuned["shape"] if tuned else "library default"},
         "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP32_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": kernel_label(eng),
+                     "traffic": traffic, "kernel": kernel_label(eng, int((tuned or {}).get("shape", {}).get("waves", 3))),
                      "kernel_ms": kern_ms, "flops_per_launch": flops_launch},
         "roofline_tally": {"bound": "hbm", "achieved": tally_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": tally_gbs / HBM_PEAK_GBS,

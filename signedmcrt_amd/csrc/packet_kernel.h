@@ -601,6 +601,8 @@ SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
 struct InterpretedSweep {
     static constexpr bool kCompiled = false;
     static constexpr int kRegions = 0;
+    static constexpr bool kDiagnostics = true;  // event trace, scatter moments, end-early (tests)
+    static constexpr bool kDetectors = true;    // detector records and the ghost continuation
     template <uint32_t KM>
     static SMCRT_DEV float region(int, const V3<float>&, const float* __restrict__) { return 0.f; }
 };
@@ -610,6 +612,7 @@ struct InterpretedSweep {
 template <bool ADJ, uint32_t KM, int WAVES = 2, class SW = InterpretedSweep>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr bool kDiag = SW::kDiagnostics, kDet = SW::kDetectors;
     CDevNode* const nodes = (CDevNode*)P.nodes;
     CKCold* const C = (CKCold*)P.cold;
     CDevRegion* const regions = (CDevRegion*)P.regions;
@@ -646,6 +649,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
 
     Lane L;
     L.st = ST_IDLE;
+    auto trace_ev = [&](int kind) {
+        if constexpr (kDiag) trace_event(C, L, kind);
+    };
     L.set = 0;
     L.layer = 0;
     L.tcount = 0;
@@ -751,7 +757,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 for (int w = 0; w < 4; ++w)
                     for (int m = 0; m < P.nmat; ++m) adj_sc(A, w, m, P.nmat) = 0.f;
             }
-            trace_event(C, L, 1);
+            trace_ev(1);
         }
 
         // ---- the sweep: every region at the lane's position (lanes not mid-walk) ----
@@ -810,7 +816,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             // src/inttau2.f90:85-97
             if (smin >= 0.f) end_kind = classify_escape(C, L);
             else if (dmin > eps) do_march = true;
-            else if (L.ghost) end_kind = classify_escape(C, L);  // outside the grid: ends at the surface
+            else if (kDet && L.ghost) end_kind = classify_escape(C, L);  // outside the grid: ends at the surface
             else if (dmin <= L.dprev || dmin <= eps_kill) cross = true;  // converged onto a surface
             else do_march = true;  // within eps but moving away from the surface: keep marching
         } else if (st0 == ST_CROSSED) {
@@ -913,10 +919,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 L.iface = true;
                 if (!rflag) {
                     L.layer = nl;
-                    trace_event(C, L, 3);
+                    trace_ev(3);
                 } else {
                     L.px = L.opx; L.py = L.opy; L.pz = L.opz;  // step back inside (:174)
-                    trace_event(C, L, 4);
+                    trace_ev(4);
                     if (++L.bounces > C->max_bounces) end_kind = END_BOUNCE;
                 }
                 if (end_kind < 0) L.st = ST_START;
@@ -933,7 +939,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             float d = dmin;
             if (W.flight_over) d = f_div(L.tau - L.taurun, kap);
             else L.taurun += t;
-            if (L.ghost) {  // outside the grid: move without a walk (nothing to tally)
+            if (kDet && L.ghost) {  // outside the grid: move without a walk (nothing to tally)
                 move(L, d);
                 if (W.flight_over) end_kind = classify_escape(C, L);
                 else L.st = ST_MARCHED;
@@ -1053,7 +1059,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 // left the grid (src/inttau2.f90:256-259,282-285): the outcome is fixed.  With
                 // detectors the packet moves on, untallied, to the end of its flight or the next
                 // surface, as the reference's loop does (:64-100), for its final position.
-                if (C->n_det > 0 && !W.flight_over) {
+                if (kDet && C->n_det > 0 && !W.flight_over) {
                     L.ghost = true;
                     L.st = ST_MARCHED;
                 } else {
@@ -1080,7 +1086,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     L.nsc++;
                     out[O_SCATTERS]++;
                     scatter(L, m.hgg, m.g2, u[1], u[2]);
-                    trace_event(C, L, 2);
+                    trace_ev(2);
                     bool alive = true;
                     if (L.weight < 0.01f) {
                         if (u[0] < 0.1f) L.weight *= 10.f;  // roulette
@@ -1103,15 +1109,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     }
                     L.nsc++;
                     out[O_SCATTERS]++;
-                    if ((C->flags & 1u) && L.nsc <= 4) {
+                    if (kDiag && (C->flags & 1u) && L.nsc <= 4) {
                         double* mo = C->moments + (L.nsc - 1) * 3;
                         atomicAdd(mo + 0, (double)L.px); atomicAdd(mo + 1, (double)L.py); atomicAdd(mo + 2, (double)L.pz);
                         atomicAdd(mo + 12, (double)L.px * L.px); atomicAdd(mo + 13, (double)L.py * L.py);
                         atomicAdd(mo + 14, (double)L.pz * L.pz);
                     }
                     scatter(L, m.hgg, m.g2, u[1], u[2]);
-                    trace_event(C, L, 2);
-                    if ((C->flags & 2u) && L.nsc >= 5) {
+                    trace_ev(2);
+                    if (kDiag && (C->flags & 2u) && L.nsc >= 5) {
                         end_kind = END_ABSORB;
                     } else {
                         L.tau = f_neglog(u[3]);  // next flight's optical depth
@@ -1149,13 +1155,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             }
             pend_idx = -1;
             pend_li = -1;
-            if (C->n_det > 0 && end_kind != END_ERR) record_detectors(C, L);
+            if (kDet && C->n_det > 0 && end_kind != END_ERR) record_detectors(C, L);
             switch (end_kind) {
-            case END_ABSORB: out[O_ABSORBED]++; trace_event(C, L, 5); break;
-            case END_R: out[O_R]++; trace_event(C, L, 6); break;
-            case END_T: out[O_T]++; trace_event(C, L, 6); break;
-            case END_SURFACE: out[O_KSURF]++; trace_event(C, L, 7); break;
-            case END_BOUNCE: out[O_KBOUNCE]++; trace_event(C, L, 7); break;
+            case END_ABSORB: out[O_ABSORBED]++; trace_ev(5); break;
+            case END_R: out[O_R]++; trace_ev(6); break;
+            case END_T: out[O_T]++; trace_ev(6); break;
+            case END_SURFACE: out[O_KSURF]++; trace_ev(7); break;
+            case END_BOUNCE: out[O_KBOUNCE]++; trace_ev(7); break;
             default: break;  // error paths are counted where they occur
             }
             if (ADJ) {

@@ -903,6 +903,8 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     "struct SceneSweep {\n"
                     "    static constexpr bool kCompiled = true;\n"
                     "    static constexpr int kRegions = " + std::to_string(s->regions.size()) + ";\n"
+                    "    static constexpr bool kDiagnostics = false;\n"
+                    "    static constexpr bool kDetectors = " + std::string(s->ddets.empty() ? "false" : "true") + ";\n"
                     "    template <uint32_t KM>\n"
                     "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel) {\n"
                     "        switch (r) {\n";
@@ -1127,8 +1129,11 @@ static bool jit_build(smcrt_scene* s) {
 
 // Run-time compiled kernels for this launch?  SMCRT_JIT=0 never, =1 always; by default for
 // launches of at least 2^22 packets (a compile costs a few seconds once per program; the disk
-// cache makes later processes load it directly).  Batches always use the library's kernels.
+// cache makes later processes load it directly).  Batches always use the library's kernels, and
+// so do launches asking for the test diagnostics (event trace, scatter moments, end-early),
+// which the compiled kernels leave out.
 static bool jit_wanted(const smcrt_run_opts* o) {
+    if (o && (o->trace_packets > 0 || (o->flags & (SMCRT_FLAG_MOMENTS | SMCRT_FLAG_END_EARLY)))) return false;
     const char* e = getenv("SMCRT_JIT");
     if (e && !strcmp(e, "0")) return false;
     if (e && !strcmp(e, "1")) return true;

@@ -103,3 +103,34 @@ def test_compiled_kernels_are_cached(gpu, monkeypatch, tmp_path):
     assert e2.kernel_variant()[0] == "basic+compiled(cached)"
     for k in KEYS:
         assert r1.stats[k] == r2.stats[k], k
+
+
+def test_compiled_kernels_with_detectors(gpu, monkeypatch, tmp_path):
+    """A scene with detectors compiles its kernels with the detector records and the ghost
+    continuation; detector counts agree with the interpreter's."""
+    import os
+    from signedmcrt_amd import config
+    toml = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "toml", "slab_dects.toml")
+    sc, _ = config.scene_from_toml(toml)
+    interp, comp = _pair(sc, monkeypatch, tmp_path)
+    a = interp.run(400_000, seed=7)
+    b = comp.run(400_000, seed=7)
+    assert "compiled" in comp.kernel_variant()[0]
+    for k in KEYS:
+        assert _close(a.stats[k], b.stats[k]), (k, a.stats[k], b.stats[k])
+    for da, db in zip(a.detectors, b.detectors):
+        assert da.sum() > 0
+        assert _close(float(da.sum()), float(db.sum())), (da.sum(), db.sum())
+
+
+def test_diagnostic_launches_use_the_library_kernels(gpu, monkeypatch, tmp_path):
+    """Event traces and scatter moments are left out of the compiled kernels: launches asking for
+    them run the library's kernels even with SMCRT_JIT=1."""
+    monkeypatch.setenv("SMCRT_CACHE_DIR", str(tmp_path))
+    monkeypatch.setenv("SMCRT_JIT", "1")
+    from signedmcrt_amd.engine import Engine
+    sc, _ = scenes.build("scat_test2")
+    eng = Engine(sc)
+    r = eng.run(1000, seed=5, tally=False, trace_packets=10, trace_events=4)
+    assert "compiled" not in eng.kernel_variant()[0]
+    assert np.all(r.trace[:, 0, 3] == 1)

@@ -178,11 +178,12 @@ SMCRT_DEV void trace_event(CKCold* C, Lane& L, int kind) {
 
 // emitters (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338.
 // The emission block: u0, u1 the emitter's draws, u2 the first flight's optical depth.
+template <int KSRC>
 SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
     float u[4];
     draw_block(L, P, u);
     L.tau = f_neglog(u[2]);
-    switch (C->src_kind) {
+    switch (KSRC >= 0 ? KSRC : C->src_kind) {
     case 2: {  // uniform: draws rx, ry
         float rx = u[0], ry = u[1];
         L.px = C->p1[0] + rx * C->p2[0] + ry * C->p3[0];
@@ -603,6 +604,10 @@ struct InterpretedSweep {
     static constexpr int kRegions = 0;
     static constexpr bool kDiagnostics = true;  // event trace, scatter moments, end-early (tests)
     static constexpr bool kDetectors = true;    // detector records and the ghost continuation
+    static constexpr int kSource = -1;          // source kind (-1: read at run time)
+    // a compiled policy overwrites the scene's fixed launch parameters (grid, epsilons, region
+    // count) with constants
+    static SMCRT_DEV void bake(KParams&) {}
     template <uint32_t KM>
     static SMCRT_DEV float region(int, const V3<float>&, const float* __restrict__) { return 0.f; }
 };
@@ -613,6 +618,7 @@ template <bool ADJ, uint32_t KM, int WAVES = 2, class SW = InterpretedSweep>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr bool kDiag = SW::kDiagnostics, kDet = SW::kDetectors;
+    SW::bake(P);
     CDevNode* const nodes = (CDevNode*)P.nodes;
     CKCold* const C = (CKCold*)P.cold;
     CDevRegion* const regions = (CDevRegion*)P.regions;
@@ -743,7 +749,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         // ---- emission ----
         if (L.st == ST_EMIT) {
             rng_init(L.rng);
-            emit(P, C, L);
+            emit<SW::kSource>(P, C, L);
             L.rx = 0.f; L.ry = 0.f; L.rz = 0.f;
             L.bounces = 0;
             L.nsc = 0;

@@ -905,6 +905,7 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     "    static constexpr int kRegions = " + std::to_string(s->regions.size()) + ";\n"
                     "    static constexpr bool kDiagnostics = false;\n"
                     "    static constexpr bool kDetectors = " + std::string(s->ddets.empty() ? "false" : "true") + ";\n"
+                    "    static constexpr int kSource = " + std::to_string(s->src.kind) + ";\n"
                     "    template <uint32_t KM>\n"
                     "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel) {\n"
                     "        switch (r) {\n";
@@ -954,7 +955,25 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
         }
         o += "            return " + (vs.empty() ? std::string("0.f") : vs.back()) + ";\n        }\n";
     }
-    o += "        }\n        return 0.f;\n    }\n};\n";
+    o += "        }\n        return 0.f;\n    }\n";
+    {  // the scene's fixed launch parameters, exactly as fill_common sets them
+        KParams P;
+        KCold Cc;
+        fill_common(s, P, Cc);
+        o += "    static SMCRT_DEV void bake(KParams& P) {\n";
+        o += "        P.nreg = " + std::to_string(P.nreg) + ";\n";
+        o += "        P.nx = " + std::to_string(P.nx) + "; P.ny = " + std::to_string(P.ny) + "; P.nz = " + std::to_string(P.nz) +
+             ";\n";
+        const std::pair<const char*, float> fl[] = {{"xmax", P.xmax}, {"ymax", P.ymax}, {"zmax", P.zmax}, {"sx", P.sx},
+                                                    {"sy", P.sy},     {"sz", P.sz},     {"eps", P.eps},   {"eps_kill", P.eps_kill}};
+        for (const auto& f : fl) {
+            o += std::string("        P.") + f.first + " = ";
+            jit_float(o, f.second);
+            o += ";\n";
+        }
+        o += "    }\n";
+    }
+    o += "};\n";
     for (int adj = 0; adj < 2; ++adj)
         for (int w = 2; w <= 3; ++w) o += "template __global__ void " + jit_name(km, adj, w).substr(7) + "(KParams);\n";
     o += "}  // namespace smcrt\n";

@@ -68,7 +68,7 @@ def kernel_label(eng, waves):
     name, scene_mask, variant_mask = eng.kernel_variant()
     if "compiled" in name:
         return "packet_kernel<false, %du, %d, SceneSweep> (%s)" % (scene_mask, waves, name)
-    return "packet_kernel<false, %du, %d> (%s)" % (variant_mask, waves, name)
+    return "packet_kernel<false, %du, %d> (%s)" % (variant_mask, min(waves, 3), name)
 
 
 def read_traffic(workload):

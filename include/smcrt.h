@@ -217,10 +217,12 @@ typedef struct smcrt_run_opts {
                               least this many are idle (1 = at once; 0 -> default)            */
     uint32_t grad_mask_lo; /* smcrt_grad: materials whose derivatives are computed, bit m =   */
     uint32_t grad_mask_hi; /* material m (0 and 0: every material); at most 16 set            */
-    int32_t waves;         /* resident wavefronts per SIMD the packet kernel is built for: 2 or
-                              3 (0 -> 3).  3 exists for the kind-specialised kernels (fewer
+    int32_t waves;         /* resident wavefronts per SIMD the packet kernel is built for: 2,
+                              3 or 4 (0 -> 3).  3 exists for the kind-specialised kernels (fewer
                               registers, a few spilled; faster where latency dominates over
-                              the tally); the general kernel always runs the 2-wave build     */
+                              the tally); the general kernel always runs the 2-wave build.
+                              4 exists only for the run-time compiled kernels (the library's
+                              run 3 instead)                                                  */
 } smcrt_run_opts;
 
 /* How a packet ended.  A+R+T+K = launched. */

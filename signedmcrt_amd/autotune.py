@@ -9,7 +9,8 @@ Knobs (`SPACE`):
   refill_min  regeneration policy: a wavefront refills its idle lanes once at least this many
               are idle (1 = at once)
   tile        LDS-privatised FP32 tally tile on/off
-  waves       resident wavefronts per SIMD the kernel is built for (2, or 3: fewer registers)
+  waves       resident wavefronts per SIMD the kernel is built for (2, 3 or 4: fewer registers;
+              4 only for the run-time compiled kernels, the library's run 3 instead)
 
 Every probe launches the same packets (fixed seed and first packet id).  Each packet draws from
 its own (seed, id) random stream, so the transported work is identical for every shape: the
@@ -39,7 +40,7 @@ SPACE: Dict[str, Tuple] = {
     "walk_cap": (1, 2, 4, 8, 16),
     "refill_min": (1, 8, 16, 32),
     "tile": (True, False),
-    "waves": (2, 3),
+    "waves": (2, 3, 4),
 }
 DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "waves": 3}
 # counters that must not change with the launch shape
@@ -174,14 +175,18 @@ def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
 
     start = {k: DEFAULT[k] for k in space}
     if n_probe is None:
-        n0 = 200_000
-        probe(start, n0)  # warm-up: the LDS tile's pilot run, code load
-        ms = probe(start, n0)
-        # a second calibration closer to the target size (short launches are fill/drain bound
-        # and would underestimate the rate)
-        n1 = int(min(max(n0 * 0.25 * target_ms / max(ms, 1e-3), n0), 50_000_000))
-        ms = probe(start, n1)
-        n_probe = int(min(max(n1 * target_ms / max(ms, 1e-3), 200_000), 50_000_000))
+        n = 200_000
+        probe(start, n)  # warm-up: the LDS tile's pilot run, code load
+        # calibrate at growing sizes until the size for target_ms is stable: short launches are
+        # fill/drain bound, and large ones run the scene's run-time compiled kernels (faster than
+        # the library's), so the rate measured at small sizes underestimates the probe's
+        for _ in range(5):
+            ms = probe(start, n)
+            n_next = int(min(max(n * target_ms / max(ms, 1e-3), 200_000), 50_000_000))
+            if abs(n_next - n) <= 0.15 * n:
+                break
+            n = min(n_next, 16 * n)
+        n_probe = n_next
     t0 = time.perf_counter()
     best, table = coordinate_descent(lambda sh: probe(sh, n_probe), space, start, sweeps, min_gain)
     best_ms = dict((tuple(sorted(s.items())), c) for s, c in table)[tuple(sorted(best.items()))]

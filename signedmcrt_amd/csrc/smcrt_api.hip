@@ -160,7 +160,7 @@ struct DevBuf {
 struct JitKernels {
     bool tried = false, ok = false, cached = false;
     hipModule_t mod = nullptr;
-    hipFunction_t fwd[2] = {nullptr, nullptr}, adj[2] = {nullptr, nullptr};  // [waves - 2]
+    hipFunction_t fwd[3] = {nullptr, nullptr, nullptr}, adj[3] = {nullptr, nullptr, nullptr};  // [waves - 2]
     std::string error;
     double seconds = 0.0;
 };
@@ -188,7 +188,7 @@ struct smcrt_scene {
     size_t det_bins = 0;
     // LDS tally tile (voxel box) for the 2- and 3-wave kernels, chosen from a pilot run on first use
     bool tile_ready = false;
-    int tile_t0[2][3] = {{0, 0, 0}, {0, 0, 0}}, tile_tn[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int tile_t0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, tile_tn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     // device
     int device = -1;
     int num_cu = 0;
@@ -858,12 +858,18 @@ static PacketKernelFn packet_kernel_for(const smcrt_scene* s, bool adj, int wave
     return (waves == 3 && k3) ? k3 : (adj ? v.adj : v.fwd);
 }
 
-// Resident waves per SIMD a launch runs with: 3 (the default) where a 3-wave build exists,
-// else 2.
+// Resident waves per SIMD a launch of the library's kernels runs with: 3 (the default; also for
+// 4, which only the run-time compiled kernels are built for) where a 3-wave build exists, else 2.
 static int launch_waves(const smcrt_scene* s, const smcrt_run_opts* o, bool adj) {
     const int w = (o && o->waves > 0) ? o->waves : 3;
     const KernelVariant& v = kernel_variant(s);
-    return (w == 3 && (adj ? v.adj3 : v.fwd3)) ? 3 : 2;
+    return (w >= 3 && (adj ? v.adj3 : v.fwd3)) ? 3 : 2;
+}
+
+// The same for the run-time compiled kernels, built for 2, 3 and 4 waves (default 3).
+static int compiled_waves(const smcrt_run_opts* o) {
+    const int w = (o && o->waves > 0) ? o->waves : 3;
+    return w < 2 ? 2 : (w > 4 ? 4 : w);
 }
 
 // ---- packet kernels compiled at run time for one scene -------------------------------------
@@ -975,7 +981,7 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
     }
     o += "};\n";
     for (int adj = 0; adj < 2; ++adj)
-        for (int w = 2; w <= 3; ++w) o += "template __global__ void " + jit_name(km, adj, w).substr(7) + "(KParams);\n";
+        for (int w = 2; w <= 4; ++w) o += "template __global__ void " + jit_name(km, adj, w).substr(7) + "(KParams);\n";
     o += "}  // namespace smcrt\n";
     return o;
 }
@@ -1002,12 +1008,15 @@ static std::string jit_cache_dir() {
 }
 
 // cache file: u32 count, then per name (u32 length, bytes), then u64 code size and the code
+// kernels in a compiled module: forward and adjoint, each built for 2, 3 and 4 waves
+static constexpr uint32_t kJitKernels = 6;
+
 static bool jit_cache_read(const std::string& path, std::vector<std::string>& names, std::vector<char>& code) {
     FILE* f = fopen(path.c_str(), "rb");
     if (!f) return false;
     bool ok = true;
     uint32_t n = 0;
-    ok = ok && fread(&n, 4, 1, f) == 1 && n == 4;
+    ok = ok && fread(&n, 4, 1, f) == 1 && n == kJitKernels;
     names.assign(ok ? n : 0, std::string());
     for (uint32_t i = 0; ok && i < n; ++i) {
         uint32_t len = 0;
@@ -1094,7 +1103,7 @@ static bool jit_build(smcrt_scene* s) {
         }
         std::vector<std::string> exprs;
         for (int adj = 0; adj < 2; ++adj)
-            for (int w = 2; w <= 3; ++w) {
+            for (int w = 2; w <= 4; ++w) {
                 exprs.push_back(jit_name(km, adj, w));
                 (void)hiprtcAddNameExpression(prog, exprs.back().c_str());
             }
@@ -1133,8 +1142,14 @@ static bool jit_build(smcrt_scene* s) {
         J.mod = nullptr;
         return false;
     }
-    hipFunction_t* fn[4] = {&J.fwd[0], &J.fwd[1], &J.adj[0], &J.adj[1]};
-    for (int i = 0; i < 4; ++i)
+    hipFunction_t* fn[kJitKernels] = {&J.fwd[0], &J.fwd[1], &J.fwd[2], &J.adj[0], &J.adj[1], &J.adj[2]};
+    if (names.size() != kJitKernels) {
+        J.error = "unexpected kernel count in the compiled module";
+        (void)hipModuleUnload(J.mod);
+        J.mod = nullptr;
+        return false;
+    }
+    for (uint32_t i = 0; i < kJitKernels; ++i)
         if (hipModuleGetFunction(fn[i], J.mod, names[i].c_str()) != hipSuccess) {
             J.error = "hipModuleGetFunction failed for " + names[i];
             (void)hipModuleUnload(J.mod);
@@ -1190,7 +1205,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
     L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
-    L.waves = compiled ? ((o && (o->waves == 2)) ? 2 : 3) : launch_waves(s, o, adj);
+    L.waves = compiled ? compiled_waves(o) : launch_waves(s, o, adj);
     const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
@@ -1279,15 +1294,16 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
     std::vector<double> h(nv);
     if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
         // the tile takes the LDS left per 256-lane block at the kernel's register occupancy
-        // (2 or 3 blocks per CU for the 2- and 3-wave builds), at most 64 KiB of FP32
+        // (2, 3 or 4 blocks per CU for the 2-, 3- and 4-wave builds), at most 64 KiB of FP32
         int lds_cu = 0;
         if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
             lds_cu <= 0)
             lds_cu = 160 * 1024;
-        for (int w = 2; w <= 3; ++w) {
-            int bpc = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, packet_kernel_for(s, false, w), 256, 0) != hipSuccess ||
-                bpc < 1)
+        for (int w = 2; w <= 4; ++w) {
+            int bpc = w;  // the 4-wave build exists only compiled at run time: 4 blocks per CU
+            if (w < 4 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, packet_kernel_for(s, false, w), 256, 0) !=
+                              hipSuccess ||
+                          bpc < 1))
                 bpc = w;
             const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
             const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
@@ -1363,7 +1379,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     const bool compiled = jit_wanted(o) && jit_build(s);
     int tile_vox = 0;
     if (use_tile) {
-        const int wi = (compiled ? (o->waves == 2 ? 2 : 3) : launch_waves(s, o, adj)) - 2;
+        const int wi = (compiled ? compiled_waves(o) : launch_waves(s, o, adj)) - 2;
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }

@@ -18,7 +18,7 @@ def _separable_cost(sh):
     c += abs(sh["walk_cap"] - 8) / 4.0 + (0.5 if sh["walk_cap"] == 8 and sh["chunk"] == 64 else 0.0)
     c += {1: 0.3, 8: 0.0, 16: 0.2, 32: 0.4}[sh["refill_min"]]
     c += 0.0 if sh["tile"] else 1.0
-    c += {2: 0.1, 3: 0.0}[sh["waves"]]
+    c += {2: 0.1, 3: 0.0, 4: 0.05}[sh["waves"]]
     return c
 
 
@@ -28,7 +28,7 @@ def test_coordinate_descent_finds_the_grid_minimum():
                 key=_separable_cost)
     assert _separable_cost(best) == pytest.approx(_separable_cost(brute))
     assert best == brute
-    # far fewer probes than the full product (3*4*5*4*2*2 = 960)
+    # far fewer probes than the full product (3*4*5*4*2*3 = 1440)
     assert len(table) < 40
     assert table[0][0] == autotune.DEFAULT
 
@@ -73,7 +73,7 @@ def test_shapes_transport_identical_packets(gpu):
     eng = Engine(sc)
     ref = eng.run(60_000, seed=5).stats
     for sh in ({"refill_min": 32}, {"refill_min": 64, "chunk": 512}, {"walk_cap": 1, "block": 64},
-               {"walk_cap": 16, "chunk": 128, "refill_min": 8}, {"waves": 2}, {"waves": 3, "block": 128}):
+               {"walk_cap": 16, "chunk": 128, "refill_min": 8}, {"waves": 2}, {"waves": 3, "block": 128}, {"waves": 4}):
         st = eng.run(60_000, seed=5, **sh).stats
         for k in autotune.INVARIANT:
             assert st[k] == ref[k], (sh, k)

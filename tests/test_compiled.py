@@ -73,10 +73,11 @@ def test_compiled_kernels_match_interpreter(gpu, name, monkeypatch, tmp_path):
     assert abs(a.jmean.sum() / b.jmean.sum() - 1) < 1e-3
     m = a.jmean > 0.05 * a.jmean.max()
     assert np.median(np.abs(a.jmean[m] / b.jmean[m] - 1)) < 1e-3
-    # 2- and 3-wave compiled builds: the same packets
-    c = comp.run(300_000, seed=23, waves=2)
-    for k in KEYS:
-        assert c.stats[k] == b.stats[k], k
+    # 2-, 3- and 4-wave compiled builds: the same packets
+    for w in (2, 4):
+        c = comp.run(300_000, seed=23, waves=w)
+        for k in KEYS:
+            assert c.stats[k] == b.stats[k], (w, k)
 
 
 def test_compiled_adjoint_matches_interpreter(gpu, monkeypatch, tmp_path):

@@ -960,83 +960,64 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         bool walk_done = W.active && !W.inside;  // started outside the grid
         bool walking = W.active && W.inside;
         const uint32_t gbase = (uint32_t)L.set * (uint32_t)P.grid_stride;
+        // per-axis step of the crossing distance (the direction is fixed over the walk)
+        const float dtx = f_rcp(fabsf(L.dx) * P.sx), dty = f_rcp(fabsf(L.dy) * P.sy), dtz = f_rcp(fabsf(L.dz) * P.sz);
+        const int sgx = L.dx > 0.f ? 1 : -1, sgy = L.dy > 0.f ? 1 : -1, sgz = L.dz > 0.f ? 1 : -1;
+        // one chord per lane and trip, written as selects rather than nested branches (lanes of a
+        // wavefront differ in which axis they step and whether they merge, flush or leave)
         for (int k = 0; k < cap; ++k) {
             if (wave_ballot(walking) == 0ull) break;
             bool gpush = false;  // this lane hands its merged chord to the global tally
             uint32_t gidx = 0u;
             float glen = 0.f;
             if (walking) {
-                {
-                    const float tn = fminf(W.tmx, fminf(W.tmy, W.tmz));
-                    const float len = fminf(tn, W.d) - W.t;
-                    if (len > 0.f) {
-                        ev[E_CROSS]++;
-                        if (ADJ) {
-                            const float w = C->weights ? C->weights[W.idx] : 1.f;
-                            W.aint += adjC * len + 0.5f * w * len * len;
-                            adjC += w * len;
-                        }
-                        if (grid) {
-                            if (W.idx == pend_idx) {
-                                pend_len += len;
-                            } else {
-                                if (pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
-                                else if (pend_idx >= 0) {
-                                    gpush = true;
-                                    gidx = gbase + (uint32_t)pend_idx;
-                                    glen = pend_len;
-                                    ev[E_ATOMICS]++;
-                                }
-                                pend_idx = W.idx;
-                                pend_len = len;
-                                const unsigned lx = (unsigned)(W.ix - P.t0[0]), ly = (unsigned)(W.iy - P.t0[1]),
-                                               lz = (unsigned)(W.iz - P.t0[2]);
-                                pend_li = (lx < (unsigned)P.tn[0] && ly < (unsigned)P.tn[1] && lz < (unsigned)P.tn[2])
-                                              ? (int)(lx + P.tn[0] * (ly + P.tn[1] * lz)) : -1;
-                            }
-                        }
-                    }
-                    if (tn >= W.d) {
-                        walk_done = true;
-                        walking = false;
-                    } else {
-                        W.t = tn;
-                        if (W.tmx <= W.tmy && W.tmx <= W.tmz) {
-                            const int s = L.dx > 0.f ? 1 : -1;
-                            W.ix += s;
-                            if ((unsigned)W.ix >= (unsigned)P.nx) {
-                                W.inside = false;
-                                walk_done = true;
-                                walking = false;
-                            } else {
-                                W.idx += s;
-                                W.tmx += f_rcp(fabsf(L.dx) * P.sx);
-                            }
-                        } else if (W.tmy <= W.tmz) {
-                            const int s = L.dy > 0.f ? 1 : -1;
-                            W.iy += s;
-                            if ((unsigned)W.iy >= (unsigned)P.ny) {
-                                W.inside = false;
-                                walk_done = true;
-                                walking = false;
-                            } else {
-                                W.idx += s * P.nx;
-                                W.tmy += f_rcp(fabsf(L.dy) * P.sy);
-                            }
-                        } else {
-                            const int s = L.dz > 0.f ? 1 : -1;
-                            W.iz += s;
-                            if ((unsigned)W.iz >= (unsigned)P.nz) {
-                                W.inside = false;
-                                walk_done = true;
-                                walking = false;
-                            } else {
-                                W.idx += s * P.nx * P.ny;
-                                W.tmz += f_rcp(fabsf(L.dz) * P.sz);
-                            }
-                        }
+                const float tn = fminf(W.tmx, fminf(W.tmy, W.tmz));
+                const float len = fminf(tn, W.d) - W.t;
+                const bool crossed = len > 0.f;
+                if (crossed) {
+                    ev[E_CROSS]++;
+                    if (ADJ) {
+                        const float w = C->weights ? C->weights[W.idx] : 1.f;
+                        W.aint += adjC * len + 0.5f * w * len * len;
+                        adjC += w * len;
                     }
                 }
+                if (grid) {
+                    // a chord in the pending voxel merges; one in another voxel hands the pending
+                    // chord to the LDS tile or the global queue and becomes pending itself
+                    const bool flush = crossed && W.idx != pend_idx;
+                    if (flush && pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
+                    gpush = flush && pend_li < 0 && pend_idx >= 0;
+                    gidx = gbase + (uint32_t)pend_idx;
+                    glen = pend_len;
+                    if (gpush) ev[E_ATOMICS]++;
+                    const unsigned lx = (unsigned)(W.ix - P.t0[0]), ly = (unsigned)(W.iy - P.t0[1]),
+                                   lz = (unsigned)(W.iz - P.t0[2]);
+                    const int li = (lx < (unsigned)P.tn[0] && ly < (unsigned)P.tn[1] && lz < (unsigned)P.tn[2])
+                                       ? (int)(lx + P.tn[0] * (ly + P.tn[1] * lz)) : -1;
+                    pend_len = flush ? len : (crossed ? pend_len + len : pend_len);
+                    pend_li = flush ? li : pend_li;
+                    pend_idx = flush ? W.idx : pend_idx;
+                }
+                // next voxel boundary: the axis with the nearest crossing (x, then y, on ties)
+                const bool ax = W.tmx <= W.tmy && W.tmx <= W.tmz;
+                const bool ay = !ax && W.tmy <= W.tmz;
+                const int c = ax ? W.ix + sgx : (ay ? W.iy + sgy : W.iz + sgz);
+                const int lim = ax ? P.nx : (ay ? P.ny : P.nz);
+                const bool ends = tn >= W.d;                  // the step ends inside this voxel
+                const bool leaves = !ends && (unsigned)c >= (unsigned)lim;  // into the grid's outside
+                const bool steps = !ends && !leaves;
+                W.t = ends ? W.t : tn;
+                W.inside = !leaves;
+                W.ix = (steps && ax) ? c : W.ix;
+                W.iy = (steps && ay) ? c : W.iy;
+                W.iz = (steps && !ax && !ay) ? c : W.iz;
+                W.idx += steps ? (ax ? sgx : (ay ? sgy * P.nx : sgz * P.nx * P.ny)) : 0;
+                W.tmx = (steps && ax) ? W.tmx + dtx : W.tmx;
+                W.tmy = (steps && ay) ? W.tmy + dty : W.tmy;
+                W.tmz = (steps && !ax && !ay) ? W.tmz + dtz : W.tmz;
+                walk_done = !steps;
+                walking = steps;
             }
             if (P.jmean) tally_push(Q, gpush, gidx, glen, tbase, lane_id);
         }

@@ -1132,24 +1132,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         uint32_t eidx = 0u;
         float elen = 0.f;
         if (end_kind >= 0) {
-            if (pend_li >= 0) {  // flush the merged chord before the lane changes packet/set
-                atomicAdd(s_tile + pend_li, pend_len);
-            } else if (pend_idx >= 0) {
-                epush = true;
-                eidx = gbase + (uint32_t)pend_idx;
-                elen = pend_len;
-                ev[E_ATOMICS]++;
-            }
+            // flush the merged chord before the lane changes packet/set
+            if (pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
+            epush = pend_li < 0 && pend_idx >= 0;
+            eidx = gbase + (uint32_t)pend_idx;
+            elen = pend_len;
+            ev[E_ATOMICS] += epush ? 1u : 0u;
             pend_idx = -1;
             pend_li = -1;
             if (kDet && C->n_det > 0 && end_kind != END_ERR) record_detectors(C, L);
-            switch (end_kind) {
-            case END_ABSORB: out[O_ABSORBED]++; trace_ev(5); break;
-            case END_R: out[O_R]++; trace_ev(6); break;
-            case END_T: out[O_T]++; trace_ev(6); break;
-            case END_SURFACE: out[O_KSURF]++; trace_ev(7); break;
-            case END_BOUNCE: out[O_KBOUNCE]++; trace_ev(7); break;
-            default: break;  // error paths are counted where they occur
+            // outcome counters as selects (error paths are counted where they occur)
+            out[O_ABSORBED] += end_kind == END_ABSORB ? 1u : 0u;
+            out[O_R] += end_kind == END_R ? 1u : 0u;
+            out[O_T] += end_kind == END_T ? 1u : 0u;
+            out[O_KSURF] += end_kind == END_SURFACE ? 1u : 0u;
+            out[O_KBOUNCE] += end_kind == END_BOUNCE ? 1u : 0u;
+            if constexpr (kDiag) {
+                switch (end_kind) {
+                case END_ABSORB: trace_ev(5); break;
+                case END_R: case END_T: trace_ev(6); break;
+                case END_SURFACE: case END_BOUNCE: trace_ev(7); break;
+                default: break;
+                }
             }
             if (ADJ) {
                 const bool isr = end_kind == END_R;

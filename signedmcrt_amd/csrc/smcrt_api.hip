@@ -967,6 +967,7 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
         KCold Cc;
         fill_common(s, P, Cc);
         o += "    static SMCRT_DEV void bake(KParams& P) {\n";
+        o += "        P.nsets = 1;  // compiled kernels run single-set launches only\n";
         o += "        P.nreg = " + std::to_string(P.nreg) + ";\n";
         o += "        P.nx = " + std::to_string(P.nx) + "; P.ny = " + std::to_string(P.ny) + "; P.nz = " + std::to_string(P.nz) +
              ";\n";

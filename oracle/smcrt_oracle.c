@@ -21,11 +21,20 @@
  *   - The reference's HG scatter draws one extra number for a disabled biased-scattering branch
  *     (p = 0, src/photon.f90:731-747); it is not drawn here.
  *
- * Parity pins (tests/test_oracle.py): SDF values against the reference's own test vectors
- * (test/SDF/test_SDF.f90), normals against analytic gradients, Fresnel/reflect/refract against the
- * flang-built reference surfaces.f90 (oracle/_ref) and the textbook equations, scatter against
- * the HG distribution moments, the DDA against brute-force chord clipping, the whole loop against
- * the reference's end-to-end KATs (test/end_to_end/test_scat.f90) and adding-doubling.
+ * Parity pins (tests/test_oracle.py, tests/test_oracle_pins.py): SDF values against the
+ * reference's own test vectors (test/SDF/test_SDF.f90); SmoothUnion, onion, twist, elongate and
+ * revolution against closed forms and identities between SDFs (band limits, k/6 on the bisector,
+ * value/slope/curvature continuity at the band edge; ||p|-R|-t; twist k = 0 and of a z-cylinder;
+ * elongated sphere = capsule / rounded box; revolved sphere = torus, revolved box = rectangle
+ * distance); normals against analytic gradients; Fresnel/reflect/refract against the flang-built
+ * reference surfaces.f90 (oracle/_ref) and the textbook equations; the interface table and Fresnel
+ * events against a clear n = 1.5 slab under a collimated beam (killed = R + (1-R) R,
+ * transmitted (1-R)^2, Snell directions, TIR, the grazing-crossing kill); the circular and uniform
+ * emitters against the reference's own circle test (test/photon/test_photon.f90:195-210) and
+ * their uniform distributions; scatter against the HG distribution moments; the DDA against
+ * brute-force chord clipping; the whole loop against the reference's end-to-end KATs
+ * (test/end_to_end/test_scat.f90), the closed-form scatter-site moments and the infinite-medium
+ * total path 1/mu_a.
  */
 #include <math.h>
 #include <stdint.h>

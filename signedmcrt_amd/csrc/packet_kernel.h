@@ -115,8 +115,10 @@ struct KParams {
     int chunk, dda_cap;
     int refill_min;  // regeneration policy: refill once at least this many lanes are idle
     // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP32 LDS per block
-    // and are flushed into the FP64 grid at the end (tn[0] == 0: no tile)
+    // and are flushed into the FP64 grid every `tile_flush` packets the block ends (0: only at
+    // the end) and at the end (tn[0] == 0: no tile)
     int t0[3], tn[3];
+    int tile_flush;
     const struct KCold* cold;  // rarely used parameters (constant memory)
 };
 
@@ -576,6 +578,21 @@ SMCRT_DEV void tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, doubl
     if (Q.n >= 64) tally_drain(Q, base, lane, false);
 }
 
+// Move the tile's FP32 sums into the FP64 grid (one wavefront, any time): each entry is taken and
+// zeroed by one LDS exchange, so chords other wavefronts add meanwhile land either in the value
+// taken or in the emptied entry, and none is lost or counted twice.
+SMCRT_DEV void tile_flush_wave(float* s_tile, int tile_vox, const KParams& P, double* __restrict__ tbase, int lane) {
+    for (int i = lane; i < tile_vox; i += 64) {
+        const float v = atomicExch(s_tile + i, 0.f);
+        if (v != 0.f) {
+            const int lx = i % P.tn[0], r = i / P.tn[0];
+            const int ly = r % P.tn[1], lz = r / P.tn[1];
+            const size_t g = (size_t)(P.t0[0] + lx) + (size_t)P.nx * ((size_t)(P.t0[1] + ly) + (size_t)P.ny * (P.t0[2] + lz));
+            unsafeAtomicAdd(tbase + g, (double)v);
+        }
+    }
+}
+
 SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
     const float gx = (L.px + P.xmax) * P.sx, gy = (L.py + P.ymax) * P.sy, gz = (L.pz + P.zmax) * P.sz;
     W.ix = (int)floorf(gx);
@@ -631,6 +648,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     float* const s_tile = reinterpret_cast<float*>(smem + sizeof(DevMat) * n_mats);
     const int tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) s_tile[i] = 0.f;
+    // packets this block has ended (paces the periodic tile flush)
+    __shared__ unsigned s_ended;
+    if (threadIdx.x == 0) s_ended = 0u;
     // this block's tally target (its XCD's partial grid; workgroups go to XCDs round-robin)
     double* const tbase = P.xgrid ? P.xgrid + (size_t)(blockIdx.x % (unsigned)P.xcopies) * P.xstride : P.jmean;
     // each wavefront's tally queue comes last (after the tile and the adjoint's scores)
@@ -1177,10 +1197,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             L.st = ST_IDLE;
         }
         if (P.jmean) tally_push(Q, epush, eidx, elen, tbase, lane_id);
+
+        // ---- periodic tile flush: each FP32 tile sum holds at most ~tile_flush packets ----
+        if (!ADJ && tile_vox > 0 && P.tile_flush > 0) {
+            const uint64_t me = wave_ballot(end_kind >= 0);
+            if (me) {
+                const unsigned k = (unsigned)__builtin_popcountll(me);
+                unsigned old = 0u;
+                if (lane_id == 0) old = atomicAdd(&s_ended, k);
+                old = __shfl(old, 0);
+                const unsigned f = (unsigned)P.tile_flush;
+                if (old / f != (old + k) / f) tile_flush_wave(s_tile, tile_vox, P, tbase, lane_id);
+            }
+        }
     }
     if (P.jmean) tally_drain(Q, tbase, lane_id, true);
 
-    // ---- flush the tally tile (single-set launches only) into the FP64 grid ----
+    // ---- flush what the tally tile (single-set launches only) holds into the FP64 grid ----
     if (tile_vox > 0) {
         __syncthreads();
         for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) {

@@ -1069,6 +1069,17 @@ static bool jit_build(smcrt_scene* s) {
     uint64_t h = fnv1a(src.data(), src.size());
     for (int i = 0; i < kJitHeaders; ++i) h = fnv1a(kJitHeaderSources[i], strlen(kJitHeaderSources[i]), h);
     for (int i = 0; i < nopts; ++i) h = fnv1a(opts[i], strlen(opts[i]), h);
+    // the toolchain and the device the code object is for: a cache written by another hipRTC,
+    // HIP runtime or GPU is not reused
+    {
+        int ver[3] = {0, 0, 0};
+        (void)hiprtcVersion(&ver[0], &ver[1]);
+        (void)hipRuntimeGetVersion(&ver[2]);
+        h = fnv1a(ver, sizeof ver, h);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, s->device) == hipSuccess)
+            h = fnv1a(prop.gcnArchName, strnlen(prop.gcnArchName, sizeof prop.gcnArchName), h);
+    }
     char hex[32];
     snprintf(hex, sizeof hex, "%016llx", (unsigned long long)h);
     const std::string path = jit_cache_dir() + "/packet_kernel_" + hex + ".co";
@@ -1276,6 +1287,11 @@ struct AdjTables {
 static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
                       hipStream_t stream, float* kernel_ms, const AdjTables* at = nullptr);
 
+// A block moves its LDS tile's FP32 sums into the FP64 grid after every this many packets it
+// ends, so no FP32 sum holds more than about this many packets' chords (relative rounding of a
+// sum of n chords: ~sqrt(n) 2^-24 typical, n 2^-24 at worst), however large the launch.
+static constexpr int kTileFlushPackets = 16384;
+
 // Pick the LDS tally tile from a pilot run of the scene (once per scene).
 static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stream) {
     if (s->tile_ready) return SMCRT_OK;
@@ -1383,6 +1399,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
         const int wi = (compiled ? compiled_waves(o) : launch_waves(s, o, adj)) - 2;
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
+        P.tile_flush = kTileFlushPackets;
     }
     LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat, compiled);
     P.chunk = sh.chunk;

@@ -341,7 +341,11 @@ int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt
                        smcrt_batch** out);
 /* One persistent launch over every set (graph replay).  Clears the grids first. */
 int smcrt_batch_run(smcrt_batch* batch);
-/* Copy set `set`'s grid (raw FP64) and stats back; jmean may be NULL. */
+/* Copy set `set`'s grid (raw FP64) and stats back; jmean may be NULL.  The outcome counters
+   (launched .. scatters) are the set's own; the event and error counters (ev_*, err_*) are those
+   of the launch of the set's geometry group, shared by every set on that geometry; kernel_ms is
+   the device time of the last replay of the whole graph (every group's launch, the grid clears
+   and the partial-grid reduction). */
 int smcrt_batch_read(smcrt_batch* batch, int32_t set, double* jmean, smcrt_stats* stats);
 int smcrt_batch_destroy(smcrt_batch* batch);
 

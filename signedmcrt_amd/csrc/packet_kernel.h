@@ -18,7 +18,7 @@
 // path-length estimator of `update_grids` (:213-292).  A walk advances at most `dda_cap` chords
 // per iteration and resumes in the next one, so a few long walks do not stall the wavefront.
 // Consecutive chords in the same voxel (a step ending and the next one starting in it) are
-// merged; a block keeps the hottest voxel box in an LDS tile, and the other chords go through a
+// merged; a block keeps the hottest voxel box in an FP64 LDS tile, and the other chords go through a
 // per-wavefront LDS queue to FP64 atomics into its XCD's partial grid.  When the flight's optical
 // depth runs out, the albedo test absorbs or scatters (HG, src/photon.f90:713-779) and a new
 // flight starts.
@@ -114,11 +114,9 @@ struct KParams {
     unsigned long long* work;              // work counter
     int chunk, dda_cap;
     int refill_min;  // regeneration policy: refill once at least this many lanes are idle
-    // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP32 LDS per block
-    // and are flushed into the FP64 grid every `tile_flush` packets the block ends (0: only at
-    // the end) and at the end (tn[0] == 0: no tile)
+    // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP64 LDS per block
+    // and are flushed into the FP64 grid at the end (tn[0] == 0: no tile)
     int t0[3], tn[3];
-    int tile_flush;
     const struct KCold* cold;  // rarely used parameters (constant memory)
 };
 
@@ -578,21 +576,6 @@ SMCRT_DEV void tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, doubl
     if (Q.n >= 64) tally_drain(Q, base, lane, false);
 }
 
-// Move the tile's FP32 sums into the FP64 grid (one wavefront, any time): each entry is taken and
-// zeroed by one LDS exchange, so chords other wavefronts add meanwhile land either in the value
-// taken or in the emptied entry, and none is lost or counted twice.
-SMCRT_DEV void tile_flush_wave(float* s_tile, int tile_vox, const KParams& P, double* __restrict__ tbase, int lane) {
-    for (int i = lane; i < tile_vox; i += 64) {
-        const float v = atomicExch(s_tile + i, 0.f);
-        if (v != 0.f) {
-            const int lx = i % P.tn[0], r = i / P.tn[0];
-            const int ly = r % P.tn[1], lz = r / P.tn[1];
-            const size_t g = (size_t)(P.t0[0] + lx) + (size_t)P.nx * ((size_t)(P.t0[1] + ly) + (size_t)P.ny * (P.t0[2] + lz));
-            unsafeAtomicAdd(tbase + g, (double)v);
-        }
-    }
-}
-
 SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
     const float gx = (L.px + P.xmax) * P.sx, gy = (L.py + P.ymax) * P.sy, gz = (L.pz + P.zmax) * P.sz;
     W.ix = (int)floorf(gx);
@@ -644,27 +627,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     DevMat* const s_mat = reinterpret_cast<DevMat*>(smem);
     const int n_mats = P.nsets * P.nreg;
     for (int i = threadIdx.x; i < n_mats; i += blockDim.x) s_mat[i] = P.mats[i];
-    // the tally tile follows the materials
-    float* const s_tile = reinterpret_cast<float*>(smem + sizeof(DevMat) * n_mats);
+    // the tally tile follows the materials (FP64 sums: the LDS adds round like FP32 register adds,
+    // and an FP32 sum of a block's ~1e4-1e5 chords into a hot voxel drifts by ~1e-5 relative)
+    double* const s_tile = reinterpret_cast<double*>(smem + sizeof(DevMat) * n_mats);
     const int tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
-    for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) s_tile[i] = 0.f;
-    // packets this block has ended (paces the periodic tile flush)
-    __shared__ unsigned s_ended;
-    if (threadIdx.x == 0) s_ended = 0u;
+    for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) s_tile[i] = 0.0;
     // this block's tally target (its XCD's partial grid; workgroups go to XCDs round-robin)
     double* const tbase = P.xgrid ? P.xgrid + (size_t)(blockIdx.x % (unsigned)P.xcopies) * P.xstride : P.jmean;
     // each wavefront's tally queue comes last (after the tile and the adjoint's scores)
     TallyQueue Q;
     Q.n = 0;
     {
-        size_t qoff = sizeof(DevMat) * n_mats + sizeof(float) * ((tile_vox + 3) & ~3);
+        size_t qoff = sizeof(DevMat) * n_mats + sizeof(double) * ((tile_vox + 1) & ~1);
         if (ADJ) qoff += sizeof(float) * 4 * P.nmat * blockDim.x + sizeof(double) * 4 * P.nmat;
         Q.e = reinterpret_cast<uint2*>(smem + qoff) + (threadIdx.x / 64) * TALLY_QUEUE;
     }
     AdjLds A;
     if (ADJ) {
         A.stride = blockDim.x;
-        unsigned char* base = smem + sizeof(DevMat) * n_mats + sizeof(float) * ((tile_vox + 3) & ~3);
+        unsigned char* base = smem + sizeof(DevMat) * n_mats + sizeof(double) * ((tile_vox + 1) & ~1);
         A.sc = reinterpret_cast<float*>(base);
         A.g = reinterpret_cast<double*>(base + sizeof(float) * 4 * P.nmat * blockDim.x);
         for (int i = threadIdx.x; i < 4 * P.nmat; i += blockDim.x) A.g[i] = 0.0;
@@ -1006,7 +987,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     // a chord in the pending voxel merges; one in another voxel hands the pending
                     // chord to the LDS tile or the global queue and becomes pending itself
                     const bool flush = crossed && W.idx != pend_idx;
-                    if (flush && pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
+                    if (flush && pend_li >= 0) atomicAdd(s_tile + pend_li, (double)pend_len);
                     gpush = flush && pend_li < 0 && pend_idx >= 0;
                     gidx = gbase + (uint32_t)pend_idx;
                     glen = pend_len;
@@ -1153,7 +1134,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         float elen = 0.f;
         if (end_kind >= 0) {
             // flush the merged chord before the lane changes packet/set
-            if (pend_li >= 0) atomicAdd(s_tile + pend_li, pend_len);
+            if (pend_li >= 0) atomicAdd(s_tile + pend_li, (double)pend_len);
             epush = pend_li < 0 && pend_idx >= 0;
             eidx = gbase + (uint32_t)pend_idx;
             elen = pend_len;
@@ -1197,32 +1178,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             L.st = ST_IDLE;
         }
         if (P.jmean) tally_push(Q, epush, eidx, elen, tbase, lane_id);
-
-        // ---- periodic tile flush: each FP32 tile sum holds at most ~tile_flush packets ----
-        if (!ADJ && tile_vox > 0 && P.tile_flush > 0) {
-            const uint64_t me = wave_ballot(end_kind >= 0);
-            if (me) {
-                const unsigned k = (unsigned)__builtin_popcountll(me);
-                unsigned old = 0u;
-                if (lane_id == 0) old = atomicAdd(&s_ended, k);
-                old = __shfl(old, 0);
-                const unsigned f = (unsigned)P.tile_flush;
-                if (old / f != (old + k) / f) tile_flush_wave(s_tile, tile_vox, P, tbase, lane_id);
-            }
-        }
     }
     if (P.jmean) tally_drain(Q, tbase, lane_id, true);
 
-    // ---- flush what the tally tile (single-set launches only) holds into the FP64 grid ----
+    // ---- flush the tally tile (single-set launches only) into the FP64 grid ----
     if (tile_vox > 0) {
         __syncthreads();
         for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) {
-            const float v = s_tile[i];
-            if (v != 0.f) {
+            const double v = s_tile[i];
+            if (v != 0.0) {
                 const int lx = i % P.tn[0], r = i / P.tn[0];
                 const int ly = r % P.tn[1], lz = r / P.tn[1];
                 const size_t g = (size_t)(P.t0[0] + lx) + (size_t)P.nx * ((size_t)(P.t0[1] + ly) + (size_t)P.ny * (P.t0[2] + lz));
-                unsafeAtomicAdd(tbase + g, (double)v);
+                unsafeAtomicAdd(tbase + g, v);
             }
         }
     }

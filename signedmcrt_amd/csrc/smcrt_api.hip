@@ -1218,7 +1218,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
     L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
     L.waves = compiled ? compiled_waves(o) : launch_waves(s, o, adj);
-    const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(float) * ((tile_vox + 3) & ~3);
+    const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(double) * ((tile_vox + 1) & ~1);
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
     if (L.block > 256) L.block = 256;
@@ -1287,11 +1287,6 @@ struct AdjTables {
 static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
                       hipStream_t stream, float* kernel_ms, const AdjTables* at = nullptr);
 
-// A block moves its LDS tile's FP32 sums into the FP64 grid after every this many packets it
-// ends, so no FP32 sum holds more than about this many packets' chords (relative rounding of a
-// sum of n chords: ~sqrt(n) 2^-24 typical, n 2^-24 at worst), however large the launch.
-static constexpr int kTileFlushPackets = 16384;
-
 // Pick the LDS tally tile from a pilot run of the scene (once per scene).
 static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stream) {
     if (s->tile_ready) return SMCRT_OK;
@@ -1311,7 +1306,7 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
     std::vector<double> h(nv);
     if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
         // the tile takes the LDS left per 256-lane block at the kernel's register occupancy
-        // (2, 3 or 4 blocks per CU for the 2-, 3- and 4-wave builds), at most 64 KiB of FP32
+        // (2, 3 or 4 blocks per CU for the 2-, 3- and 4-wave builds), at most 64 KiB of FP64 sums
         int lds_cu = 0;
         if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
             lds_cu <= 0)
@@ -1325,7 +1320,7 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
             const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
             const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
             const long per_block = (long)lds_cu / bpc - mats - queue - 256;
-            const int budget = (int)std::max(0L, std::min(16384L, per_block / (long)sizeof(float)));
+            const int budget = (int)std::max(0L, std::min(8192L, per_block / (long)sizeof(double)));
             if (budget > 0)
                 choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0[w - 2], s->tile_tn[w - 2]);
         }
@@ -1399,7 +1394,6 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
         const int wi = (compiled ? compiled_waves(o) : launch_waves(s, o, adj)) - 2;
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
-        P.tile_flush = kTileFlushPackets;
     }
     LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat, compiled);
     P.chunk = sh.chunk;
@@ -1649,6 +1643,8 @@ struct smcrt_batch {
     DevBuf d_mats, d_grids, d_stats, d_info, d_work, d_cold, d_xgrid;
     size_t total_doubles = 0, total_stats = 0;
     int copies = 0;  // per-XCD partial grids (0: none)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the last replay
+    float replay_ms = 0.f;
 };
 
 extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, const smcrt_batch_set* sets,
@@ -1820,8 +1816,13 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
 extern "C" int smcrt_batch_run(smcrt_batch* b) {
     if (!b) return fail(SMCRT_EINVAL, "null batch");
     HIPCHECK(hipSetDevice(b->device));
+    if (!b->ev0) HIPCHECK(hipEventCreate(&b->ev0));
+    if (!b->ev1) HIPCHECK(hipEventCreate(&b->ev1));
+    HIPCHECK(hipEventRecord(b->ev0, b->stream));
     HIPCHECK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHECK(hipEventRecord(b->ev1, b->stream));
     HIPCHECK(hipStreamSynchronize(b->stream));
+    HIPCHECK(hipEventElapsedTime(&b->replay_ms, b->ev0, b->ev1));
     return SMCRT_OK;
 }
 
@@ -1834,10 +1835,24 @@ extern "C" int smcrt_batch_read(smcrt_batch* b, int32_t set, double* jmean, smcr
         HIPCHECK(hipMemcpy(jmean, b->d_grids.as<double>() + b->set_grid_off[set], b->set_nv[set] * sizeof(double),
                            hipMemcpyDeviceToHost));
     if (st) {
-        unsigned long long h[NOUT];
+        unsigned long long h[NOUT], e[NEV];
         HIPCHECK(hipMemcpy(h, b->d_stats.as<unsigned long long>() + g.stats_off + (size_t)b->set_slot[set] * NOUT,
                            sizeof h, hipMemcpyDeviceToHost));
+        // the group's event counters follow its sets' outcome counters
+        HIPCHECK(hipMemcpy(e, b->d_stats.as<unsigned long long>() + g.stats_off + g.sets.size() * NOUT, sizeof e,
+                           hipMemcpyDeviceToHost));
         memset(st, 0, sizeof *st);
+        st->ev_sweeps = e[E_SWEEPS];
+        st->ev_region_evals = e[E_REGION];
+        st->ev_normals = e[E_NORMALS];
+        st->ev_steps = e[E_STEPS];
+        st->ev_crossings = e[E_CROSS];
+        st->ev_interfaces = e[E_IFACE];
+        st->ev_fresnel = e[E_FRESNEL];
+        st->err_unreachable = e[E_UNREACH];
+        st->err_nan = e[E_NAN];
+        st->ev_atomics = e[E_ATOMICS];
+        st->kernel_ms = b->replay_ms;
         st->launched = h[O_LAUNCHED];
         st->absorbed = h[O_ABSORBED];
         st->escaped_r = h[O_R];
@@ -1860,6 +1875,8 @@ extern "C" int smcrt_batch_destroy(smcrt_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
     b->d_mats.release();
     b->d_grids.release();
     b->d_stats.release();

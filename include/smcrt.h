@@ -295,6 +295,26 @@ int smcrt_run(smcrt_scene* scene, const smcrt_run_opts* opts, smcrt_tallies* out
 int smcrt_run_device(smcrt_scene* scene, const smcrt_run_opts* opts, double* d_jmean,
                      smcrt_stats* stats, void* stream);
 
+/* Bytes of the device array the allocation-free entry point works in (one per GPU, held from the
+   library's load until the process ends). */
+#define SMCRT_SCRATCH_BYTES (64 << 20)
+
+/* smcrt_run_device without any allocation (no counterpart in the reference): for callers that
+   cannot allocate device memory — code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc.  Same packets, counters and fluence as smcrt_run_device (up to FP64
+   summation order); the caller passes the grid (d_jmean) and the stream (NULL = default stream).
+   Every other buffer — counters, launch parameters, the scene's program, materials and polygon
+   accelerator, and as many per-XCD partial grids as fit — lives in a file-scope device array of
+   SMCRT_SCRATCH_BYTES per GPU; a scene whose program and accelerator do not fit is rejected
+   (SMCRT_ERANGE), a grid whose partial copies do not fit is tallied directly.  It runs the
+   scene's run-time compiled kernels only if an earlier call built them on this device, and the
+   LDS tile only if an earlier call chose it.  Event tracing is rejected (SMCRT_EINVAL); detector
+   bins are not recorded; stats->kernel_ms is 0.  With stats == NULL the call only enqueues work
+   on `stream` and returns; with stats it waits for the launch.  Calls that overlap on one GPU share
+   the array and race: serialise them. */
+int smcrt_run_device_noalloc(smcrt_scene* scene, const smcrt_run_opts* opts, double* d_jmean,
+                             smcrt_stats* stats, void* stream);
+
 /* SDF evaluation on the device: the reference's `sdf%evaluate` (src/sdfs/sdf_base.f90:251-259)
    and `calcNormal` (:165-189) for region `region` at n host points (xyz interleaved).
    dist[n] and/or normals[3n] may be NULL.  FP32 on the device; normals are exact gradients

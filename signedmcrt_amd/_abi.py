@@ -64,6 +64,7 @@ class Source(C.Structure):
 
 DET_CIRCLE, DET_ANNULUS, DET_CAMERA = 1, 2, 3
 MAX_DETECTORS = 16
+SCRATCH_BYTES = 64 << 20  # the allocation-free entry point's device array per GPU
 
 
 class Detector(C.Structure):
@@ -160,7 +161,8 @@ STRUCTS = {"node": Node, "region": Region, "material": Material, "grid": Grid,
 
 # every symbol include/smcrt.h declares (checked by tests/test_abi.py)
 EXPORTS = ["smcrt_abi_version", "smcrt_last_error", "smcrt_sizeof", "smcrt_scene_create", "smcrt_scene_destroy",
-           "smcrt_scene_set_materials", "smcrt_run", "smcrt_run_device", "smcrt_evaluate",
+           "smcrt_scene_set_materials", "smcrt_run", "smcrt_run_device", "smcrt_run_device_noalloc",
+           "smcrt_evaluate",
            "smcrt_grad", "smcrt_batch_create", "smcrt_batch_run", "smcrt_batch_read",
            "smcrt_batch_destroy", "smcrt_normalise_fluence", "smcrt_write_nrrd",
            "smcrt_detector_bins", "smcrt_config_load", "smcrt_write_detector", "smcrt_render",
@@ -195,6 +197,7 @@ def lib() -> C.CDLL:
         "smcrt_scene_set_materials": (C.c_int, [C.c_void_p, P(Material), C.c_int32]),
         "smcrt_run": (C.c_int, [C.c_void_p, P(RunOpts), P(Tallies)]),
         "smcrt_run_device": (C.c_int, [C.c_void_p, P(RunOpts), C.c_void_p, P(Stats), C.c_void_p]),
+        "smcrt_run_device_noalloc": (C.c_int, [C.c_void_p, P(RunOpts), C.c_void_p, P(Stats), C.c_void_p]),
         "smcrt_evaluate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, P(C.c_float),
                                      P(C.c_float), P(C.c_float)]),
         "smcrt_grad": (C.c_int, [C.c_void_p, P(RunOpts), P(C.c_float), P(C.c_double), P(GradOut)]),

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -200,6 +201,11 @@ struct smcrt_scene {
     DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
         d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold, d_det, d_det_counts, d_accel,
         d_adj_mats, d_adj_mus, d_xgrid;
+    // host copies of the region-indexed material table and per-material mu_s the allocation-free
+    // entry point uploads from (kept alive across its asynchronous copies; rewritten only when
+    // the materials change)
+    std::vector<DevMat> na_mats;
+    std::vector<float> na_mus;
 };
 
 // Per-node constants the device would otherwise recompute (in VALU) at every evaluation, from
@@ -1885,6 +1891,223 @@ extern "C" int smcrt_batch_destroy(smcrt_batch* b) {
     b->d_cold.release();
     b->d_xgrid.release();
     delete b;
+    return SMCRT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// allocation-free packet loop (smcrt_run_device_noalloc)
+// ------------------------------------------------------------------------------------------
+// Every buffer the packet loop needs beyond the caller's grid lives in one file-scope device array
+// (each GPU holds its own copy, at its own address, from the library's load until the process
+// ends): the work counter, outcome/event counters, moments, set table, cold parameters, the scene
+// program, materials and polygon accelerator, and as many per-XCD partial grids as fit.  In host
+// code `scratch` names a host-side stand-in; its device address comes from hipGetSymbolAddress,
+// once per device (kept in g_scratch).  Calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) unsigned char scratch[SMCRT_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch[256];  // device address of `scratch` per device (nullptr: not yet)
+static int g_scratch_cu[256], g_scratch_xcc[256];
+
+static int scratch_for_device(int dev, unsigned char** base, int* num_cu, int* n_xcc) {
+    if (dev < 0 || dev >= 256) return fail(SMCRT_EDEVICE, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SMCRT_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        int cu = 0, xcc = 1;
+        HIPCHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || xcc < 1) xcc = 1;
+        g_scratch_cu[dev] = cu;
+        g_scratch_xcc[dev] = xcc;
+        g_scratch[dev] = (unsigned char*)ptr;
+    }
+    *base = g_scratch[dev];
+    *num_cu = g_scratch_cu[dev];
+    *n_xcc = g_scratch_xcc[dev];
+    return SMCRT_OK;
+}
+
+// The launch's cold parameters and set table, written into the scratch by a kernel from its
+// by-value argument (no host-to-device copy), and the counters it clears.
+struct NoallocInit {
+    KCold cold;
+    unsigned long long setinfo[4];
+};
+__global__ void k_noalloc_init(NoallocInit in, KCold* cold, unsigned long long* setinfo, unsigned long long* ctl,
+                               int n_ctl) {
+    const int t = threadIdx.x;
+    if (t == 0) *cold = in.cold;
+    if (t < 4) setinfo[t] = in.setinfo[t];
+    for (int i = t; i < n_ctl; i += blockDim.x) ctl[i] = 0ull;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" int smcrt_run_device_noalloc(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, smcrt_stats* st,
+                                        void* stream) {
+    if (!s || !o) return fail(SMCRT_EINVAL, "null scene or options");
+    if (o->trace_packets != 0 || o->trace_events != 0)
+        return fail(SMCRT_EINVAL, "event tracing needs a trace buffer: use smcrt_run");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    unsigned char* base = nullptr;
+    int num_cu = 0, n_xcc = 1;
+    int rc = scratch_for_device(dev, &base, &num_cu, &n_xcc);
+    if (rc) return rc;
+    if ((o->flags & SMCRT_FLAG_NO_TALLY)) d_jmean = nullptr;
+    if (o->n_packets == 0) {
+        if (st) memset(st, 0, sizeof *st);
+        return SMCRT_OK;
+    }
+    hipStream_t strm = (hipStream_t)stream;
+
+    // the region-indexed material table and per-material mu_s (host copies the uploads read)
+    {
+        std::vector<DevMat> dm(s->regions.size());
+        region_mats(s, s->mats.data(), dm.data());
+        if (s->na_mats.size() != dm.size() || memcmp(s->na_mats.data(), dm.data(), dm.size() * sizeof(DevMat)))
+            s->na_mats = dm;
+        std::vector<float> mus(s->mats.size());
+        for (size_t i = 0; i < s->mats.size(); ++i) mus[i] = (float)s->mats[i].mus;
+        if (s->na_mus != mus) s->na_mus = mus;
+    }
+
+    // scratch layout
+    const size_t n_ctl = 8 + (NOUT + NEV) + 24 + 1;  // work, stats, moments, tally sum
+    size_t off = 0;
+    const size_t o_ctl = off;       off = align256(off + n_ctl * sizeof(unsigned long long));
+    const size_t o_set = off;       off = align256(off + 4 * sizeof(unsigned long long));
+    const size_t o_cold = off;      off = align256(off + sizeof(KCold));
+    const size_t o_nodes = off;     off = align256(off + s->nodes.size() * sizeof(DevNode));
+    const size_t o_regions = off;   off = align256(off + s->regions.size() * sizeof(DevRegion));
+    const size_t o_mats = off;      off = align256(off + s->na_mats.size() * sizeof(DevMat));
+    const size_t o_mus = off;       off = align256(off + s->na_mus.size() * sizeof(float));
+    const size_t o_accel = off;     off = align256(off + s->accel.size() * sizeof(float));
+    const size_t fixed = off;
+    if (fixed > (size_t)SMCRT_SCRATCH_BYTES)
+        return fail(SMCRT_ERANGE,
+                    "the scene needs %zu bytes of scratch (program %zu, polygon accelerator %zu); the allocation-free "
+                    "entry point has %d",
+                    fixed, s->nodes.size() * sizeof(DevNode), s->accel.size() * sizeof(float), SMCRT_SCRATCH_BYTES);
+    // per-XCD partial grids: as many as fit in the rest (fewer than 2: tally straight into d_jmean)
+    const size_t nvox = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
+    int copies = 0;
+    if (d_jmean && !(o->flags & SMCRT_FLAG_SHARED_GRID) && n_xcc >= 2) {
+        const size_t fit = ((size_t)SMCRT_SCRATCH_BYTES - fixed) / (nvox * sizeof(double));
+        copies = (int)std::min(fit, (size_t)n_xcc);
+        if (copies < 2) copies = 0;
+    }
+    unsigned long long* ctl = (unsigned long long*)(base + o_ctl);
+    unsigned long long* setinfo = (unsigned long long*)(base + o_set);
+    KCold* d_cold = (KCold*)(base + o_cold);
+    double* xgrid = copies ? (double*)(base + fixed) : nullptr;
+
+    // the scene's program, materials and accelerator (the scene's host vectors outlive the copies)
+    HIPCHECK(hipMemcpyAsync(base + o_nodes, s->nodes.data(), s->nodes.size() * sizeof(DevNode), hipMemcpyHostToDevice, strm));
+    HIPCHECK(hipMemcpyAsync(base + o_regions, s->regions.data(), s->regions.size() * sizeof(DevRegion),
+                            hipMemcpyHostToDevice, strm));
+    HIPCHECK(hipMemcpyAsync(base + o_mats, s->na_mats.data(), s->na_mats.size() * sizeof(DevMat), hipMemcpyHostToDevice, strm));
+    HIPCHECK(hipMemcpyAsync(base + o_mus, s->na_mus.data(), s->na_mus.size() * sizeof(float), hipMemcpyHostToDevice, strm));
+    if (!s->accel.empty())
+        HIPCHECK(hipMemcpyAsync(base + o_accel, s->accel.data(), s->accel.size() * sizeof(float), hipMemcpyHostToDevice, strm));
+
+    KParams P;
+    NoallocInit init;
+    memset(&init, 0, sizeof init);
+    KCold& Cc = init.cold;
+    const int saved_cu = s->num_cu;
+    s->num_cu = num_cu;  // pick_shape sizes the persistent grid from it
+    fill_common(s, P, Cc);
+    P.nodes = (const DevNode*)(base + o_nodes);
+    P.regions = (const DevRegion*)(base + o_regions);
+    P.accel = s->accel.empty() ? nullptr : (const float*)(base + o_accel);
+    P.mats = (const DevMat*)(base + o_mats);
+    Cc.mat_mus = (const float*)(base + o_mus);
+    P.jmean = d_jmean;
+    P.xgrid = xgrid;
+    P.xstride = copies ? nvox : 0;
+    P.xcopies = copies;
+    P.seed_lo = (uint32_t)o->seed;
+    P.seed_hi = (uint32_t)(o->seed >> 32);
+    P.total = o->n_packets;
+    P.nsets = 1;
+    init.setinfo[0] = 0ull;
+    init.setinfo[1] = o->n_packets;
+    init.setinfo[2] = o->first_packet;
+    Cc.set_prefix = setinfo;
+    Cc.set_first = setinfo + 2;
+    P.work = ctl;
+    Cc.stats = ctl + 8;
+    Cc.moments = (double*)(ctl + 8 + NOUT + NEV);
+    Cc.flags = o->flags;
+    Cc.max_bounces = o->max_bounces > 0 ? o->max_bounces : 1000;
+    Cc.n_det = 0;  // detector counts are not returned by the device-grid entry points
+    P.cold = d_cold;
+    // the scene's compiled kernels when they are already built on this device (building them
+    // loads a module, which allocates); the library's otherwise.  The LDS tile when one has been
+    // chosen (its pilot run allocates a grid); none otherwise.
+    const bool compiled = jit_wanted(o) && s->jit.ok && s->device == dev;
+    const bool use_tile = d_jmean && !(o->flags & SMCRT_FLAG_NO_LDS_TILE) && s->tile_ready;
+    int tile_vox = 0;
+    if (use_tile) {
+        const int wi = (compiled ? compiled_waves(o) : launch_waves(s, o, false)) - 2;
+        for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
+        tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
+    }
+    LaunchShape sh = pick_shape(s, o, false, 1, tile_vox, P.nmat, compiled);
+    s->num_cu = saved_cu;
+    P.chunk = sh.chunk;
+    P.dda_cap = sh.walk_cap;
+    P.refill_min = sh.refill_min;
+
+    k_noalloc_init<<<1, 64, 0, strm>>>(init, d_cold, setinfo, ctl, (int)n_ctl);
+    HIPCHECK(hipGetLastError());
+    if (copies) {
+        k_fill_f64<<<(unsigned)fill_blocks((size_t)copies * nvox), 256, 0, strm>>>(xgrid, (size_t)copies * nvox);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(launch_packet_kernel(s, false, sh, strm, P));
+    HIPCHECK(hipGetLastError());
+    if (copies) {
+        k_reduce_copies<<<(unsigned)fill_blocks(nvox), 256, 0, strm>>>(d_jmean, xgrid, nvox, copies);
+        HIPCHECK(hipGetLastError());
+    }
+    if (st) {
+        double* d_sum = (double*)(ctl + 8 + NOUT + NEV + 24);
+        if (d_jmean) {
+            k_sum_f64<<<(unsigned)fill_blocks(nvox), 256, 0, strm>>>(d_jmean, nvox, d_sum);
+            HIPCHECK(hipGetLastError());
+        }
+        unsigned long long h[8 + NOUT + NEV + 24 + 1];
+        HIPCHECK(hipMemcpyAsync(h, ctl, sizeof h, hipMemcpyDeviceToHost, strm));
+        HIPCHECK(hipStreamSynchronize(strm));
+        const unsigned long long* c = h + 8;
+        const unsigned long long* e = c + NOUT;
+        memset(st, 0, sizeof *st);
+        st->launched = c[O_LAUNCHED];
+        st->absorbed = c[O_ABSORBED];
+        st->escaped_r = c[O_R];
+        st->escaped_t = c[O_T];
+        st->killed_surface = c[O_KSURF];
+        st->killed_bounce = c[O_KBOUNCE];
+        st->scatters = c[O_SCATTERS];
+        st->ev_sweeps = e[E_SWEEPS];
+        st->ev_region_evals = e[E_REGION];
+        st->ev_normals = e[E_NORMALS];
+        st->ev_steps = e[E_STEPS];
+        st->ev_crossings = e[E_CROSS];
+        st->ev_interfaces = e[E_IFACE];
+        st->ev_fresnel = e[E_FRESNEL];
+        st->err_unreachable = e[E_UNREACH];
+        st->err_nan = e[E_NAN];
+        st->ev_atomics = e[E_ATOMICS];
+        memcpy(st->moments, e + NEV, sizeof st->moments);
+        st->kernel_ms = 0.0;  // no timing events (creating them allocates)
+        if (d_jmean) memcpy(&st->tally_sum, e + NEV + 24, sizeof(double));
+    }
     return SMCRT_OK;
 }
 

@@ -50,34 +50,78 @@ WORKLOADS = {
 }
 
 
-def per_packet_rel_sd(batches):
-    """Median over voxels with fluence >= 1% of max of the per-packet relative sd, from
-    equally sized batch tallies (list of (grid, n_packets))."""
+# Voxels the per-packet fluence error is taken over: fluence >= this fraction of the maximum.  A
+# pencil beam's fluence falls four decades within a few voxels of its axis (1 % of the maximum
+# keeps 150-470 voxels of skin + vessels and the slab, too few for a stable median), so the floor
+# is set per workload to keep >= 1e4 voxels (the same floors as the per-voxel parity tests).
+ERROR_FLOOR = {"slab": 1e-3, "refracting_sphere": 1e-2, "skin_vessels": 3e-4, "polygon_regions": 1e-2}
+
+
+def per_packet_rel_sd(batches, floor):
+    """Median over voxels with fluence >= floor x max of the per-packet relative sd, from
+    equally sized batch tallies (list of (grid, n_packets)); also the number of voxels."""
     import numpy as np
-    a = np.stack([g / n for g, n in batches])
-    m = a.mean(axis=0)
-    mask = m >= 0.01 * m.max()
-    sd = a[:, mask].std(axis=0, ddof=1) / m[mask]
+    s = s2 = None
+    for g, n in batches:
+        x = g / n
+        s = x.copy() if s is None else s + x
+        s2 = x * x if s2 is None else s2 + x * x
+    k = len(batches)
+    m = s / k
+    var = np.maximum(s2 / k - m * m, 0.0) * k / (k - 1)
+    mask = m >= floor * m.max()
+    sd = np.sqrt(var[mask]) / m[mask]
     n_b = batches[0][1]
-    return float(np.median(sd) * np.sqrt(n_b))
+    return float(np.median(sd) * np.sqrt(n_b)), int(mask.sum())
 
 
-def kernel_label(eng, waves):
-    """The forward kernel's name as rocprof reports it: the library's variant, or the kernel
-    compiled at run time for the scene's program (instantiated on the scene's exact kinds)."""
+def atomic_ceiling(nv, n=1 << 26, reps=3):
+    """Measured FP64 global-atomic ceiling: n adds at uniformly random addresses of an nv-double
+    grid (torch's index_add_, one atomic per element), best of `reps`, in atomics/s."""
+    import torch
+    idx = torch.randint(0, nv, (n,), device="cuda", dtype=torch.int64)
+    src = torch.ones(n, device="cuda", dtype=torch.float64)
+    g = torch.zeros(nv, device="cuda", dtype=torch.float64)
+    g.index_add_(0, idx, src)  # warm
+    best = float("inf")
+    for _ in range(reps):
+        g.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.index_add_(0, idx, src)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    assert g.sum().item() == n
+    del idx, src, g
+    return n / (best * 1e-3)
+
+
+def kernel_label(eng, waves, n_packets):
+    """The forward kernel a launch of n_packets ran, named as rocprof reports it: the library's
+    variant, or the kernel compiled at run time for the scene's program (instantiated on the
+    scene's exact kinds), which launches of >= 2^22 packets use once it is built (SMCRT_JIT=1:
+    every launch, =0: none)."""
     name, scene_mask, variant_mask = eng.kernel_variant()
-    if "compiled" in name:
+    jit = os.environ.get("SMCRT_JIT", "")
+    if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= (1 << 22))):
         return "packet_kernel<false, %du, %d, SceneSweep> (%s)" % (scene_mask, waves, name)
-    return "packet_kernel<false, %du, %d> (%s)" % (variant_mask, min(waves, 3), name)
+    return "packet_kernel<false, %du, %d, InterpretedSweep> (%s)" % (variant_mask, min(waves, 3), name)
 
 
-def read_traffic(workload):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+def read_traffic(workload, label):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload, when the
+    summary's kernel is the one this run timed (its `kernel:` line equals `label` up to the
+    variant note in parentheses); None otherwise."""
     p = os.path.join(ROOT, "profiles", f"pmc_packet_kernel_{workload}.md")
     if not os.path.exists(p):
         return None
-    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
-    return float(m.group(1)) if m else None
+    text = open(p).read()
+    k = re.search(r"^kernel:\s*(.+?)\s*$", text, re.M)
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    if not (k and m) or k.group(1).split(" (")[0] != label.split(" (")[0]:
+        return None
+    return float(m.group(1))
 
 
 def main():
@@ -98,7 +142,7 @@ def main():
                     help="adjoint launches over the first timed step's packets (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-packets", type=int, default=2_000_000)
-    ap.add_argument("--error-batches", type=int, default=10)
+    ap.add_argument("--error-batches", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy (PMC byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -168,10 +212,14 @@ def main():
     achieved_tf = flops_launch / (kern_ms * 1e-3) / 1e12
     tally_launch = roofline.tally_bytes(tot) / args.steps
     tally_gbs = tally_launch / (kern_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.workload)
+    label = kernel_label(eng, int((tuned or {}).get("shape", {}).get("waves", 3)), n)
+    traffic = read_traffic(args.workload, label)
+    ceiling = atomic_ceiling(nv)
+    atomics_launch = tot["ev_atomics"] / args.steps
 
     # per-packet fluence error of the GPU estimator: B independent batches
     B = args.error_batches
+    floor = ERROR_FLOOR[args.workload]
     sd_gpu = None
     if B >= 2:
         nb = max(n // B, 100_000)
@@ -180,7 +228,7 @@ def main():
             gg = torch.zeros(nv, dtype=torch.float64, device="cuda")
             eng.run_device(nb, gg.data_ptr(), seed=99, first_packet=10**9 + b * nb)
             gb.append((gg.cpu().numpy(), nb))
-        sd_gpu = per_packet_rel_sd(gb)
+        sd_gpu, nvox_err = per_packet_rel_sd(gb, floor)
         del gb
 
     # gradient cost: the adjoint launch over exactly the packets of the first timed step (same
@@ -239,11 +287,12 @@ def main():
             r = pyoracle.run(scene, nbc, seed=4321, first_packet=b * nbc, threads=threads)
             secs += r["seconds"]
             cb.append((r["jmean"].reshape(-1, order="F"), nbc))
-        sd_cpu = per_packet_rel_sd(cb)
+        sd_cpu, _ = per_packet_rel_sd(cb, floor)
         cpu_rate = nbc * B / secs
         cpu = {"value": cpu_rate, "unit": "packets/s", "cores": threads, "kind": "port",
                "sample": f"{nbc * B} packets of the same workload in {B} batches, FP64 C "
-                         f"restatement of the reference loop (oracle/), OpenMP on {threads} "
+                         f"restatement of the reference loop (oracle/; the CPU port, not the "
+                         f"reference, which cannot be built here), OpenMP on {threads} "
                          f"host threads, packet loop only (omp_get_wtime)",
                "per_packet_rel_sd": sd_cpu,
                "speedup_packets": value / cpu_rate,
@@ -267,18 +316,25 @@ def main():
                    "launch_shape": tuned["shape"] if tuned else "library default"},
         "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tf / FP32_PEAK_TFLOPS,
-                     "traffic": traffic, "kernel": kernel_label(eng, int((tuned or {}).get("shape", {}).get("waves", 3))),
+                     "traffic": traffic, "kernel": label,
                      "kernel_ms": kern_ms, "flops_per_launch": flops_launch},
         "roofline_tally": {"bound": "hbm", "achieved": tally_gbs, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": tally_gbs / HBM_PEAK_GBS,
                            "bytes_per_launch": tally_launch,
                            "crossings_per_launch": tot["ev_crossings"] / args.steps},
+        "roofline_atomics": {"bound": "fp64 global atomics", "achieved": atomics_launch / (kern_ms * 1e-3),
+                             "peak": ceiling, "unit": "atomics/s",
+                             "frac": atomics_launch / (kern_ms * 1e-3) / ceiling,
+                             "atomics_per_launch": atomics_launch,
+                             "peak_from": "measured in this run: 2^26 FP64 adds at random addresses of a "
+                                          "grid of this workload's size (torch index_add_)"},
         "events_per_packet": {k: tot[k] / tot["launched"] for k in
                               ("scatters", "ev_sweeps", "ev_steps", "ev_crossings",
-                               "ev_interfaces", "ev_normals")},
+                               "ev_interfaces", "ev_normals", "ev_atomics")},
         "fractions": {k: tot[k] / tot["launched"] for k in
                       ("absorbed", "escaped_r", "escaped_t", "killed_surface")},
         "per_packet_rel_sd": sd_gpu,
+        "error_voxels": {"floor": floor, "count": nvox_err if B >= 2 else None},
         "cpu_baseline": cpu,
         "grad": grad,
         "autotune": tuned,
@@ -349,6 +405,58 @@ def bench_batched(args):
     assert launched == total, (launched, total)
     frac = {k: sum(x[k] for x in st) / total for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface")}
     budgets = np.array([s.n_packets for s in sets])
+
+    # roofline of the replay: counted FLOPs of each geometry group's launch (its event counters,
+    # its sets' scatters and packets) over the replay's device time (both packet-kernel nodes back
+    # to back, plus the grid clears and the partial-grid reduction)
+    from signedmcrt_amd import roofline
+    flops = tally = 0.0
+    for geo, eng in enumerate(engines):
+        idx = [i for i, s in enumerate(sets) if s.scene == geo]
+        if not idx:
+            continue
+        g = dict(st[idx[0]])  # ev_* and err_*: the group's
+        for k in ("launched", "scatters"):
+            g[k] = sum(st[i][k] for i in idx)
+        assert g["err_unreachable"] == 0 and g["err_nan"] == 0
+        flops += roofline.flops(eng.scene, g)
+        tally += roofline.tally_bytes(g)
+    replay_ms = st[0]["kernel_ms"]
+    achieved = flops / (replay_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+            "kernel": "packet_kernel<false, %du, %d, InterpretedSweep> x %d geometry groups (one graph replay)" % (
+                engines[0].kernel_variant()[2], min(int(shape.get("waves", 3)), 3), len(engines)),
+            "kernel_ms": replay_ms, "flops_per_launch": flops}
+    roof_tally = {"bound": "hbm", "achieved": tally / (replay_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": tally / (replay_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "bytes_per_launch": tally}
+
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on every set at 1/200 of its budget (at least 200 packets): the same mix of
+        # sets as the replay, weighted by budget
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import copy
+        import pyoracle
+        from signedmcrt_amd.scene import Scene
+        threads = len(os.sched_getaffinity(0))
+        secs, packets = 0.0, 0
+        for i, s in enumerate(sets):
+            sc = engines[s.scene].scene
+            tmp = Scene(copy.deepcopy(sc.array), sc.grid, sc.source)
+            for j, m in enumerate(tmp.materials):
+                m.mus, m.mua, m.hgg, m.n = s.mus[j], s.mua[j], s.hgg[j], s.n[j]
+            tmp = Scene(tmp.array, tmp.grid, tmp.source)
+            nn = max(s.n_packets // 200, 200)
+            r = pyoracle.run(tmp, nn, seed=4321, first_packet=s.first_packet, threads=threads)
+            secs += r["seconds"]
+            packets += nn
+        cpu = {"value": packets / secs, "unit": "packets/s", "cores": threads, "kind": "port",
+               "sample": f"{packets} packets: every set at 1/200 of its budget (>= 200), FP64 C "
+                         f"restatement of the reference loop (oracle/; the CPU port, not the "
+                         f"reference), OpenMP on {threads} host threads, packet loop only",
+               "speedup_packets": value / (packets / secs)}
     out = {
         "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -358,7 +466,8 @@ def bench_batched(args):
                    "grids": "64^3 FP64 per set", "geometries": ["slab", "refracting_sphere"],
                    "launch": "one HIP graph: fill kernels + one persistent packet-kernel node per geometry",
                    "launch_shape": shape or "library default"},
-        "fractions": frac, "autotune": tuned, "roofline": None, "cpu_baseline": None,
+        "fractions": frac, "autotune": tuned, "roofline": roof, "roofline_tally": roof_tally,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -260,16 +260,19 @@ def main():
         from signedmcrt_amd.inverse import fit_mua
         truth = np.array([scene.materials[k].mua for k in range(16)])
         nf = args.fit_packets
-        meas = np.zeros(nv)
-        eng.run_device(nf, d_grid.zero_().data_ptr(), seed=4242)
-        meas = d_grid.cpu().numpy().reshape(g.shape, order="F") / nf
+        nm = 5 * nf  # the "measurement": the engine's own tally at the true mu_a, 5x the fit's packets
+        eng.run_device(nm, d_grid.zero_().data_ptr(), seed=4242)
+        meas = d_grid.cpu().numpy().reshape(g.shape, order="F") / nm
         fscene, _ = scenes.build(args.workload, mua=[0.5] * 16)
         feng = Engine(fscene, shape=eng.shape)
         t_fit = time.perf_counter()
         hist = fit_mua(feng, meas, materials=list(range(16)), mua0=0.5, n_packets=nf,
-                       iters=args.fit_iters, lr=0.1, block=10)
+                       iters=args.fit_iters, lr=0.1, lr_final=0.02,
+                       average_last=max(1, args.fit_iters // 5), block=10)
         err = np.abs(hist.final / truth - 1)
-        fit = {"iters": args.fit_iters, "packets_per_launch": nf, "seconds": time.perf_counter() - t_fit,
+        fit = {"iters": args.fit_iters, "packets_per_launch": nf, "measurement_packets": nm,
+               "optimizer": "Adam on log mu_a, lr 0.1 -> 0.02 (cosine), mean of the last 20 % of iterates",
+               "seconds": time.perf_counter() - t_fit,
                "loss_first": hist.loss[0], "loss_last": hist.loss[-1],
                "rel_err_start": float(np.median(np.abs(0.5 / truth - 1))),
                "rel_err_median": float(np.median(err)), "rel_err_max": float(err.max()),

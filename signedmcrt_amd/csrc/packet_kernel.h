@@ -298,10 +298,12 @@ SMCRT_DEV T prim_node(CDevNode* nodes, int i, const V3<T>& p, const float* __res
 #pragma unroll
     for (int j = 0; j < 8; ++j) qq[j] = n->q[j];
     V3<T> q = p;
-    if (!n->ident) {
+    if (n->ident == 0) {
 #pragma unroll
         for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
         q = xform<T>(tt, p);
+    } else if (n->ident == 2) {  // translation only
+        q = V3<T>{p.x + n->t[9], p.y + n->t[10], p.z + n->t[11]};
     }
     return prim_eval<T, KM>(n->kind, pp, qq, q, accel);
 }
@@ -321,11 +323,15 @@ SMCRT_DEV bool fast_prim_gradient(CDevNode* nodes, int i, const V3<float>& p, V3
     if (n->op != SMCRT_OP_PRIM || !(sphere || box)) return false;
     float tt[12];
     V3<float> q = p;
+    // the gradient is pulled back through the 3x3 part only: none for the identity and for a
+    // translation
     const bool ident = n->ident != 0;
-    if (!ident) {
+    if (n->ident == 0) {
 #pragma unroll
         for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
         q = xform<float>(tt, p);
+    } else if (n->ident == 2) {
+        q = V3<float>{p.x + n->t[9], p.y + n->t[10], p.z + n->t[11]};
     }
     V3<float> gq;
     if (sphere) {

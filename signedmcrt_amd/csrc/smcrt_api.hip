@@ -223,6 +223,12 @@ static void derive_node(DevNode& dn, const smcrt_node& n) {
         dn.q[4] = (float)(1.0 / baba);
         dn.q[5] = (float)(n.p[6] * baba);
         dn.q[6] = (float)(0.5 * baba);
+        // b - a along one coordinate axis (the other two components exactly 0): the cylinder's
+        // axis-aligned evaluation
+        if (n.kind == SMCRT_CYLINDER && baba > 0.0) {
+            const int nz = (bx != 0.0) + (by != 0.0) + (bz != 0.0);
+            if (nz == 1) dn.q[7] = bx != 0.0 ? 1.f : (by != 0.0 ? 2.f : 3.f);
+        }
     } else if (n.op == SMCRT_OP_CSG && n.kind == SMCRT_SMOOTH_UNION) {
         dn.q[0] = (float)(1.0 / n.k);
         dn.q[1] = (float)(n.k / 6.0);
@@ -564,13 +570,18 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
                     return fail(SMCRT_ERANGE, "polygon accelerators exceed 2^24 floats");
                 }
             }
-            bool ident = true;
+            bool ident = true, linear_ident = true;
             static const double I12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
             for (int j = 0; j < 12; ++j) {
                 dn.t[j] = (float)n.t[j];
-                if (n.t[j] != I12[j]) ident = false;
+                if (n.t[j] != I12[j]) {
+                    ident = false;
+                    if (j < 9) linear_ident = false;
+                }
             }
-            dn.ident = ident ? 1 : 0;
+            // 1: identity; 2: translation only (q = p + t, the same rounding as the full
+            // product for finite p: the unit and zero products are exact); 0: general
+            dn.ident = ident ? 1 : (linear_ident ? 2 : 0);
             s->nodes.push_back(dn);
         }
         if (nv != 1 || np != 1) {
@@ -932,9 +943,18 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                 jit_array(o, "P" + id, n.p, 8);
                 jit_array(o, "Q" + id, n.q, 8);
                 std::string q = ps.back();
-                if (!n.ident) {
+                if (n.ident == 0) {
                     jit_array(o, "T" + id, n.t, 12);
                     o += "            const V3<float> x" + id + " = xform<float>(T" + id + ", " + q + ");\n";
+                    q = "x" + id;
+                } else if (n.ident == 2) {  // translation only
+                    o += "            const V3<float> x" + id + "{" + q + ".x + ";
+                    jit_float(o, n.t[9]);
+                    o += ", " + q + ".y + ";
+                    jit_float(o, n.t[10]);
+                    o += ", " + q + ".z + ";
+                    jit_float(o, n.t[11]);
+                    o += "};\n";
                     q = "x" + id;
                 }
                 o += "            const float v" + id + " = prim_eval<float, KM>(" + kind + ", P" + id + ", Q" + id + ", " +

@@ -20,7 +20,7 @@
 struct alignas(16) DevNode {
     int op;       // SMCRT_OP_*
     int kind;     // primitive / CSG / modifier kind
-    int ident;    // 1: PRIM transform is the identity (skip the 4x3 multiply)
+    int ident;    // PRIM transform: 1 the identity (skipped), 2 a translation only (3 adds), 0 general
     float k;      // CSG smoothing
     float p[8];
     float t[12];
@@ -270,9 +270,24 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const float* __rest
         return g_sqrt(a * a + p.y * p.y) - P[1];
     }
     case 4: if constexpr (!KM_HAS(KM, KM_PRIM(4))) break; {  // cylinder(a, b, r)
-        // Q: b - a, |b - a|^2, 1 / |b - a|^2, r |b - a|^2, |b - a|^2 / 2
+        // Q: b - a, |b - a|^2, 1 / |b - a|^2, r |b - a|^2, |b - a|^2 / 2, axis (1-3 when b - a
+        // lies along x, y or z, else 0)
         const float bax = Q[0], bay = Q[1], baz = Q[2], baba = Q[3], hb = Q[6];
         V3<T> pa = vsubc(p, P[0], P[1], P[2]);
+        if (Q[7] != 0.f) {
+            // b - a along one axis (length L = |b - a|): the same distance with the projections
+            // taken directly — X = (distance from the axis) - r, Y = |along - L/2| - L/2 are the
+            // general formula's x / |b - a|^2 and y / |b - a|
+            const int ax = (int)Q[7];
+            const T u = ax == 1 ? pa.y : pa.x, v = ax == 3 ? pa.y : pa.z, w = ax == 1 ? pa.x : (ax == 2 ? pa.y : pa.z);
+            const float L = ax == 1 ? bax : (ax == 2 ? bay : baz);  // signed length along the axis
+            const T X = g_sqrt(u * u + v * v) - P[6];
+            const T Y = g_abs(w - 0.5f * L) - 0.5f * fabsf(L);
+            const float xv = val(X), yv = val(Y);
+            if (fmaxf(xv, yv) < 0.f) return g_max(X, Y);
+            const T mx = g_max0(X), my = g_max0(Y);
+            return g_sqrt(mx * mx + my * my);
+        }
         T paba = pa.x * bax + pa.y * bay + pa.z * baz;
         V3<T> w{pa.x * baba - paba * bax, pa.y * baba - paba * bay, pa.z * baba - paba * baz};
         T x = vlen(w) - Q[5];
@@ -432,10 +447,12 @@ SMCRT_DEV T region_eval(CDevNode* nodes, int first, int count, const V3<T>& p0, 
         for (int j = 0; j < 8; ++j) qq[j] = n->q[j];
         if (op == 1) {  // PRIM
             V3<T> q = q0;
-            if (!n->ident) {
+            if (n->ident == 0) {
 #pragma unroll
                 for (int j = 0; j < 12; ++j) tt[j] = n->t[j];
                 q = xform<T>(tt, q0);
+            } else if (n->ident == 2) {  // translation only
+                q = V3<T>{q0.x + n->t[9], q0.y + n->t[10], q0.z + n->t[11]};
             }
             T d = prim_eval<T, KM>(kind, pp, qq, q, accel);
             v3 = v2; v2 = v1; v1 = v0; v0 = d;

@@ -124,6 +124,8 @@ struct Lane {
     float px, py, pz, dx, dy, dz;
     float opx, opy, opz;  // position before the crossing step
     float tau, taurun, dlay_old, dprev;
+    float dfn_old;        // the pre-crossing sweep's first negative region (fn_old) and its distance
+    int fn_old;
     float weight;         // packet weight (the weight_scatter estimator; 1 otherwise)
     float rx, ry, rz;     // rounding error of the march moves so far (see move_axis)
     int layer, st, bounces, nsc, set, tcount;
@@ -571,15 +573,18 @@ SMCRT_DEV void tally_drain(TallyQueue& Q, double* __restrict__ base, int lane, b
     }
 }
 
-SMCRT_DEV void tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, double* __restrict__ base, int lane) {
+// Returns the number of entries appended (wave-uniform).
+SMCRT_DEV unsigned tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, double* __restrict__ base, int lane) {
     const uint64_t m = __ballot(want);
-    if (m == 0ull) return;
+    if (m == 0ull) return 0u;
     if (want) {
         const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         Q.e[Q.n + (int)rank] = make_uint2(idx, __float_as_uint(v));
     }
-    Q.n += __builtin_popcountll(m);
+    const unsigned cnt = (unsigned)__builtin_popcountll(m);
+    Q.n += (int)cnt;
     if (Q.n >= 64) tally_drain(Q, base, lane, false);
+    return cnt;
 }
 
 SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
@@ -676,10 +681,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     int pend_idx = -1;  // merged chord waiting for its atomic (grid index)
     int pend_li = -1;   // its tile index, -1 outside the tile
     float pend_len = 0.f;
+    // Outcome counters: per lane in the library's kernels (a lane's set can differ from its
+    // neighbours' in batched launches), wave-uniform ballot counts in single-set compiled ones.
+    constexpr bool kWaveOut = SW::kCompiled;
     unsigned out[NOUT];
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) out[i] = 0;
-    unsigned ev[NEV];  // wave-uniform except steps/crossings/atomics/errors (per lane)
+    unsigned ev[NEV];  // event counters: wave-uniform (ballot counts at converged points)
 #pragma unroll
     for (int i = 0; i < NEV; ++i) ev[i] = 0;
     int cur_set = 0;
@@ -754,6 +762,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         double* __restrict__ grid = P.jmean ? P.jmean + (size_t)L.set * P.grid_stride : nullptr;
 
         // ---- emission ----
+        if constexpr (kWaveOut) out[O_LAUNCHED] += __builtin_popcountll(wave_ballot(L.st == ST_EMIT));
         if (L.st == ST_EMIT) {
             rng_init(L.rng);
             emit<SW::kSource>(P, C, L);
@@ -764,7 +773,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             L.iters = 0;
             L.ghost = false;
             L.weight = 1.f;
-            out[O_LAUNCHED]++;
+            if constexpr (!kWaveOut) out[O_LAUNCHED]++;
             if (ADJ) {
                 adjC = 0.f;
                 for (int w = 0; w < 4; ++w)
@@ -812,8 +821,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         }
 
         int end_kind = runaway ? END_ERR : -1;  // >= 0: the packet ends this iteration
-        if (runaway) ev[E_NAN]++;
+        ev[E_NAN] += __builtin_popcountll(wave_ballot(runaway));
         bool want_dstmp = false, want_normal = false, do_march = false, cross = false, fres = false;
+        bool unreach = false;  // the reference's "This should not be reached!" branch
         int use_region = 0;
         const int st0 = (sweeping && !runaway) ? L.st : ST_IDLE;
         if (st0 == ST_START) {
@@ -856,7 +866,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     end_kind = classify_escape(C, L);
                 } else {
                     end_kind = END_ERR;  // "This should not be reached!" (:156)
-                    ev[E_UNREACH]++;
+                    unreach = true;
                 }
             } else {
                 L.layer = nl;  // index-matched: switch region, keep tau (:182-184)
@@ -865,21 +875,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             }
         }
         ev[E_FRESNEL] += __builtin_popcountll(wave_ballot(fres));
+        ev[E_UNREACH] += __builtin_popcountll(wave_ballot(unreach));
 
         if (cross) {
             // interface: step a little past the surface (src/inttau2.f90:104-116)
             L.opx = L.px; L.opy = L.py; L.opz = L.pz;
             L.dlay_old = dlay;
+            L.fn_old = fn;
+            L.dfn_old = dfirst;
             const float s = dmin + 2.f * eps;
             L.px += s * L.dx; L.py += s * L.dy; L.pz += s * L.dz;
             L.st = ST_CROSSED;
         }
         ev[E_IFACE] += __builtin_popcountll(wave_ballot(cross));
 
-        // dstmp(new): region nl at the pre-crossing position (waterfall over distinct regions)
-        if (const uint64_t md = wave_ballot(want_dstmp)) {
-            ev[E_REGION] += __builtin_popcountll(md);
-            const float dt = eval_lane_region<float, KM>(nodes, regions, want_dstmp, fn, V3<float>{L.opx, L.opy, L.opz}, P.accel);
+        // dstmp(new): region nl at the pre-crossing position.  The sweep there found its first
+        // negative region fn_old: regions before it were >= 0 (only the sign of dstmp matters
+        // for them), and fn_old's own distance was kept; other regions are evaluated again
+        // (waterfall over distinct regions).
+        if (wave_ballot(want_dstmp)) {
+            float dt = fn < L.fn_old ? 0.f : L.dfn_old;
+            const bool need = want_dstmp && fn > L.fn_old;
+            if (const uint64_t mn = wave_ballot(need)) {
+                ev[E_REGION] += __builtin_popcountll(mn);
+                const float de = eval_lane_region<float, KM>(nodes, regions, need, fn, V3<float>{L.opx, L.opy, L.opz}, P.accel);
+                if (need) dt = de;
+            }
             if (want_dstmp) {
                 const float dl = dlay, dn = dfirst;
                 if (dt < 0.f) {
@@ -892,11 +913,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     end_kind = classify_escape(C, L);
                 } else {
                     end_kind = END_ERR;  // "This should not be reached!" (:156)
-                    ev[E_UNREACH]++;
+                    unreach = true;
                 }
             }
         }
 
+        ev[E_UNREACH] += __builtin_popcountll(wave_ballot(unreach && want_dstmp));  // set by the dstmp branch
         // exact gradient of the normal's region (one forward-mode pass; flat folds lazily)
         V3<float> gxyz{0.f, 0.f, 0.f};
         if (wave_ballot(want_normal))
@@ -943,8 +965,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         }
 
         // ---- march step (src/inttau2.f90:64-83): optical depth, then the voxel walk ----
+        ev[E_STEPS] += __builtin_popcountll(wave_ballot(do_march));
         if (do_march) {
-            ev[E_STEPS]++;
             L.dprev = dmin;
             const float kap = mt[L.layer].kappa;
             const float t = dmin * kap;
@@ -975,14 +997,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         for (int k = 0; k < cap; ++k) {
             if (wave_ballot(walking) == 0ull) break;
             bool gpush = false;  // this lane hands its merged chord to the global tally
+            bool crossed = false;
             uint32_t gidx = 0u;
             float glen = 0.f;
             if (walking) {
                 const float tn = fminf(W.tmx, fminf(W.tmy, W.tmz));
                 const float len = fminf(tn, W.d) - W.t;
-                const bool crossed = len > 0.f;
+                crossed = len > 0.f;
                 if (crossed) {
-                    ev[E_CROSS]++;
                     if (ADJ) {
                         const float w = C->weights ? C->weights[W.idx] : 1.f;
                         W.aint += adjC * len + 0.5f * w * len * len;
@@ -997,7 +1019,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     gpush = flush && pend_li < 0 && pend_idx >= 0;
                     gidx = gbase + (uint32_t)pend_idx;
                     glen = pend_len;
-                    if (gpush) ev[E_ATOMICS]++;
                     const unsigned lx = (unsigned)(W.ix - P.t0[0]), ly = (unsigned)(W.iy - P.t0[1]),
                                    lz = (unsigned)(W.iz - P.t0[2]);
                     const int li = (lx < (unsigned)P.tn[0] && ly < (unsigned)P.tn[1] && lz < (unsigned)P.tn[2])
@@ -1026,10 +1047,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 walk_done = !steps;
                 walking = steps;
             }
-            if (P.jmean) tally_push(Q, gpush, gidx, glen, tbase, lane_id);
+            ev[E_CROSS] += __builtin_popcountll(wave_ballot(crossed));
+            if (P.jmean) ev[E_ATOMICS] += tally_push(Q, gpush, gidx, glen, tbase, lane_id);
         }
 
         // ---- step finished: move, then end of flight (albedo test) or next march step ----
+        bool scat = false;   // this lane's packet scattered
         bool wpush = false;  // weight_scatter deposit for the global tally
         uint32_t widx = 0u;
         float wlen = 0.f;
@@ -1075,10 +1098,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                         wpush = true;
                         widx = gbase + (uint32_t)(vx + P.nx * (vy + P.ny * vz));
                         wlen = wa;
-                        ev[E_ATOMICS]++;
                     }
                     L.nsc++;
-                    out[O_SCATTERS]++;
+                    scat = true;
                     scatter(L, m.hgg, m.g2, u[1], u[2]);
                     trace_ev(2);
                     bool alive = true;
@@ -1102,7 +1124,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                         adj_sc(A, 3, m.mat, P.nmat) += adjC;
                     }
                     L.nsc++;
-                    out[O_SCATTERS]++;
+                    scat = true;
                     if (kDiag && (C->flags & 1u) && L.nsc <= 4) {
                         double* mo = C->moments + (L.nsc - 1) * 3;
                         atomicAdd(mo + 0, (double)L.px); atomicAdd(mo + 1, (double)L.py); atomicAdd(mo + 2, (double)L.pz);
@@ -1127,13 +1149,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             }
         }
 
-        if (P.jmean) tally_push(Q, wpush, widx, wlen, tbase, lane_id);
+        if constexpr (kWaveOut) out[O_SCATTERS] += __builtin_popcountll(wave_ballot(scat));
+        else out[O_SCATTERS] += scat ? 1u : 0u;
+        if (P.jmean) ev[E_ATOMICS] += tally_push(Q, wpush, widx, wlen, tbase, lane_id);
 
         // non-finite guard
-        if (end_kind < 0 && L.st != ST_IDLE && !W.active &&
-            !(fabsf(L.px) < 1e30f && fabsf(L.py) < 1e30f && fabsf(L.pz) < 1e30f)) {
-            ev[E_NAN]++;
-            end_kind = END_ERR;
+        const bool nonfinite = end_kind < 0 && L.st != ST_IDLE && !W.active &&
+                               !(fabsf(L.px) < 1e30f && fabsf(L.py) < 1e30f && fabsf(L.pz) < 1e30f);
+        ev[E_NAN] += __builtin_popcountll(wave_ballot(nonfinite));
+        if (nonfinite) end_kind = END_ERR;
+        if constexpr (kWaveOut) {
+            out[O_ABSORBED] += __builtin_popcountll(wave_ballot(end_kind == END_ABSORB));
+            out[O_R] += __builtin_popcountll(wave_ballot(end_kind == END_R));
+            out[O_T] += __builtin_popcountll(wave_ballot(end_kind == END_T));
+            out[O_KSURF] += __builtin_popcountll(wave_ballot(end_kind == END_SURFACE));
+            out[O_KBOUNCE] += __builtin_popcountll(wave_ballot(end_kind == END_BOUNCE));
         }
         bool epush = false;  // the packet's last merged chord for the global tally
         uint32_t eidx = 0u;
@@ -1144,16 +1174,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             epush = pend_li < 0 && pend_idx >= 0;
             eidx = gbase + (uint32_t)pend_idx;
             elen = pend_len;
-            ev[E_ATOMICS] += epush ? 1u : 0u;
             pend_idx = -1;
             pend_li = -1;
             if (kDet && C->n_det > 0 && end_kind != END_ERR) record_detectors(C, L);
             // outcome counters as selects (error paths are counted where they occur)
-            out[O_ABSORBED] += end_kind == END_ABSORB ? 1u : 0u;
-            out[O_R] += end_kind == END_R ? 1u : 0u;
-            out[O_T] += end_kind == END_T ? 1u : 0u;
-            out[O_KSURF] += end_kind == END_SURFACE ? 1u : 0u;
-            out[O_KBOUNCE] += end_kind == END_BOUNCE ? 1u : 0u;
+            if constexpr (!kWaveOut) {
+                out[O_ABSORBED] += end_kind == END_ABSORB ? 1u : 0u;
+                out[O_R] += end_kind == END_R ? 1u : 0u;
+                out[O_T] += end_kind == END_T ? 1u : 0u;
+                out[O_KSURF] += end_kind == END_SURFACE ? 1u : 0u;
+                out[O_KBOUNCE] += end_kind == END_BOUNCE ? 1u : 0u;
+            }
             if constexpr (kDiag) {
                 switch (end_kind) {
                 case END_ABSORB: trace_ev(5); break;
@@ -1183,7 +1214,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             }
             L.st = ST_IDLE;
         }
-        if (P.jmean) tally_push(Q, epush, eidx, elen, tbase, lane_id);
+        if (P.jmean) ev[E_ATOMICS] += tally_push(Q, epush, eidx, elen, tbase, lane_id);
     }
     if (P.jmean) tally_drain(Q, tbase, lane_id, true);
 
@@ -1204,16 +1235,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     // ---- flush counters ----
 #pragma unroll
     for (int i = 0; i < NOUT; ++i)
-        if (out[i]) atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
+        if (out[i] && (!kWaveOut || lane_id == 0))
+            atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
     unsigned long long* evg = C->stats + (size_t)P.nsets * NOUT;
 #pragma unroll
-    for (int i = 0; i < NEV; ++i) {
-        unsigned long long v = ev[i];
-        const bool per_lane = (i == E_STEPS || i == E_CROSS || i == E_ATOMICS || i == E_NAN || i == E_UNREACH);
-        if (per_lane)
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane_id == 0 && v) atomicAdd(evg + i, v);
-    }
+    for (int i = 0; i < NEV; ++i)
+        if (lane_id == 0 && ev[i]) atomicAdd(evg + i, (unsigned long long)ev[i]);
     if (ADJ) {
         __syncthreads();  // every wave has left the packet loop: the block's sums are final
         for (int i = threadIdx.x; i < 4 * P.nmat; i += blockDim.x)

@@ -34,13 +34,15 @@ from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 from . import _abi as A
 
+# knobs in the order coordinate descent visits them: resident waves first (the register budget
+# changes the kernel's code, and with it which values of the other knobs are best)
 SPACE: Dict[str, Tuple] = {
+    "waves": (2, 3, 4),
     "block": (64, 128, 256),
+    "tile": (True, False),
     "chunk": (64, 128, 256, 512),
     "walk_cap": (1, 2, 4, 8, 16),
     "refill_min": (1, 8, 16, 32),
-    "tile": (True, False),
-    "waves": (2, 3, 4),
 }
 DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "waves": 3}
 # counters that must not change with the launch shape

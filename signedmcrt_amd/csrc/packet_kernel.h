@@ -615,6 +615,7 @@ struct InterpretedSweep {
     static constexpr int kRegions = 0;
     static constexpr bool kDiagnostics = true;  // event trace, scatter moments, end-early (tests)
     static constexpr bool kDetectors = true;    // detector records and the ghost continuation
+    static constexpr bool kWeight = true;       // the weight_scatter estimator (SMCRT_FLAG_WEIGHT)
     static constexpr int kSource = -1;          // source kind (-1: read at run time)
     // a compiled policy overwrites the scene's fixed launch parameters (grid, epsilons, region
     // count) with constants
@@ -628,7 +629,7 @@ struct InterpretedSweep {
 template <bool ADJ, uint32_t KM, int WAVES = 2, class SW = InterpretedSweep>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    constexpr bool kDiag = SW::kDiagnostics, kDet = SW::kDetectors;
+    constexpr bool kDiag = SW::kDiagnostics, kDet = SW::kDetectors, kWeight = SW::kWeight;
     SW::bake(P);
     CDevNode* const nodes = (CDevNode*)P.nodes;
     CKCold* const C = (CKCold*)P.cold;
@@ -772,7 +773,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             L.tcount = 0;
             L.iters = 0;
             L.ghost = false;
-            L.weight = 1.f;
+            if constexpr (kWeight) L.weight = 1.f;
             if constexpr (!kWaveOut) out[O_LAUNCHED]++;
             if (ADJ) {
                 adjC = 0.f;
@@ -936,7 +937,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 float u[4];
                 draw_block(L, P, u);  // the Fresnel event's block
                 bool rflag = u[0] <= R;  // reflect or refract
-                L.weight *= R;                // on either branch (src/inttau2.f90:167)
+                if constexpr (kWeight) L.weight *= R;  // on either branch (src/inttau2.f90:167)
                 if (rflag) {
                     float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
                     L.dx -= c * nx; L.dy -= c * ny; L.dz -= c * nz;
@@ -1082,7 +1083,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 } else {
                     end_kind = classify_escape(C, L);
                 }
-            } else if (W.flight_over && (C->flags & 16u)) {
+            } else if (kWeight && W.flight_over && (C->flags & 16u)) {
                 // weight_scatter (src/kernelsMod.f90:83-111): deposit the absorbed fraction in the
                 // interaction voxel (leaving the grid there ends the packet), scatter, then
                 // Russian roulette below THRESHOLD = 0.01 with CHANCE = 0.1 (src/constants.f90)

@@ -928,6 +928,7 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     "    static constexpr int kRegions = " + std::to_string(s->regions.size()) + ";\n"
                     "    static constexpr bool kDiagnostics = false;\n"
                     "    static constexpr bool kDetectors = " + std::string(s->ddets.empty() ? "false" : "true") + ";\n"
+                    "    static constexpr bool kWeight = false;\n"
                     "    static constexpr int kSource = " + std::to_string(s->src.kind) + ";\n"
                     "    template <uint32_t KM>\n"
                     "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel) {\n"
@@ -1109,6 +1110,13 @@ static bool jit_build(smcrt_scene* s) {
     char hex[32];
     snprintf(hex, sizeof hex, "%016llx", (unsigned long long)h);
     const std::string path = jit_cache_dir() + "/packet_kernel_" + hex + ".co";
+    // SMCRT_JIT_DUMP=1: also write the generated source beside the code object (inspection)
+    if (const char* dump = getenv("SMCRT_JIT_DUMP"); dump && !strcmp(dump, "1")) {
+        if (FILE* f = fopen((jit_cache_dir() + "/packet_kernel_" + hex + ".hip").c_str(), "w")) {
+            fwrite(src.data(), 1, src.size(), f);
+            fclose(f);
+        }
+    }
     std::vector<std::string> names;
     std::vector<char> code;
     if (jit_cache_read(path, names, code)) {
@@ -1202,10 +1210,11 @@ static bool jit_build(smcrt_scene* s) {
 // Run-time compiled kernels for this launch?  SMCRT_JIT=0 never, =1 always; by default for
 // launches of at least 2^22 packets (a compile costs a few seconds once per program; the disk
 // cache makes later processes load it directly).  Batches always use the library's kernels, and
-// so do launches asking for the test diagnostics (event trace, scatter moments, end-early),
-// which the compiled kernels leave out.
+// so do launches asking for the test diagnostics (event trace, scatter moments, end-early) or the
+// weight_scatter estimator, which the compiled kernels leave out.
 static bool jit_wanted(const smcrt_run_opts* o) {
-    if (o && (o->trace_packets > 0 || (o->flags & (SMCRT_FLAG_MOMENTS | SMCRT_FLAG_END_EARLY)))) return false;
+    if (o && (o->trace_packets > 0 || (o->flags & (SMCRT_FLAG_MOMENTS | SMCRT_FLAG_END_EARLY | SMCRT_FLAG_WEIGHT))))
+        return false;
     const char* e = getenv("SMCRT_JIT");
     if (e && !strcmp(e, "0")) return false;
     if (e && !strcmp(e, "1")) return true;

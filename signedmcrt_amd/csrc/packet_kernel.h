@@ -312,6 +312,33 @@ SMCRT_DEV T prim_node(CDevNode* nodes, int i, const V3<T>& p, const float* __res
 
 SMCRT_DEV V3<float> dual_grad(const Dual& d) { return V3<float>{d.x, d.y, d.z}; }
 
+// Closed-form gradient of a sphere (radius unused) or a box (half sizes hx, hy, hz) at q in the
+// primitive's frame: the derivative the Dual evaluation of the same expressions gives, branch for
+// branch (|x| has slope +1 at 0, max keeps its first argument on ties, sqrt has slope 0 at 0).
+SMCRT_DEV V3<float> closed_form_gradient(bool sphere, float hx, float hy, float hz, const V3<float>& q) {
+    if (sphere) {
+        const float s = f_sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+        const float h = s > 0.f ? f_rcp(s) : 0.f;
+        return V3<float>{q.x * h, q.y * h, q.z * h};
+    }
+    const float sx = q.x < 0.f ? -1.f : 1.f, sy = q.y < 0.f ? -1.f : 1.f, sz = q.z < 0.f ? -1.f : 1.f;
+    const float dx = fabsf(q.x) - hx, dy = fabsf(q.y) - hy, dz = fabsf(q.z) - hz;
+    const float mx = dx > 0.f ? dx : 0.f, my = dy > 0.f ? dy : 0.f, mz = dz > 0.f ? dz : 0.f;
+    const float s = f_sqrt(mx * mx + my * my + mz * mz);
+    const float h = s > 0.f ? f_rcp(s) : 0.f;
+    V3<float> gq{sx * mx * h, sy * my * h, sz * mz * h};
+    // min(max(dx, max(dy, dz)), 0): the largest component when negative
+    const bool zy = dz > dy;
+    const float inner = zy ? dz : dy;
+    const bool ix = inner > dx;
+    if ((ix ? inner : dx) < 0.f) {
+        if (!ix) gq.x += sx;
+        else if (zy) gq.z += sz;
+        else gq.y += sy;
+    }
+    return gq;
+}
+
 // Closed-form gradient of a single sphere or box node at p (the derivative the Dual evaluation
 // of the same expressions gives, branch for branch: |x| has slope +1 at 0, max keeps its first
 // argument on ties, sqrt has slope 0 at 0), pulled back through the node's affine transform.
@@ -335,28 +362,7 @@ SMCRT_DEV bool fast_prim_gradient(CDevNode* nodes, int i, const V3<float>& p, V3
     } else if (n->ident == 2) {
         q = V3<float>{p.x + n->t[9], p.y + n->t[10], p.z + n->t[11]};
     }
-    V3<float> gq;
-    if (sphere) {
-        const float s = f_sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
-        const float h = s > 0.f ? f_rcp(s) : 0.f;
-        gq = V3<float>{q.x * h, q.y * h, q.z * h};
-    } else {
-        const float sx = q.x < 0.f ? -1.f : 1.f, sy = q.y < 0.f ? -1.f : 1.f, sz = q.z < 0.f ? -1.f : 1.f;
-        const float dx = fabsf(q.x) - n->p[0], dy = fabsf(q.y) - n->p[1], dz = fabsf(q.z) - n->p[2];
-        const float mx = dx > 0.f ? dx : 0.f, my = dy > 0.f ? dy : 0.f, mz = dz > 0.f ? dz : 0.f;
-        const float s = f_sqrt(mx * mx + my * my + mz * mz);
-        const float h = s > 0.f ? f_rcp(s) : 0.f;
-        gq = V3<float>{sx * mx * h, sy * my * h, sz * mz * h};
-        // min(max(dx, max(dy, dz)), 0): the largest component when negative
-        const bool zy = dz > dy;
-        const float inner = zy ? dz : dy;
-        const bool ix = inner > dx;
-        if ((ix ? inner : dx) < 0.f) {
-            if (!ix) gq.x += sx;
-            else if (zy) gq.z += sz;
-            else gq.y += sy;
-        }
-    }
+    const V3<float> gq = closed_form_gradient(sphere, n->p[0], n->p[1], n->p[2], q);
     if (ident) {
         g = gq;
     } else {  // q_j = sum_i p_i t[3i + j] + t[9 + j]
@@ -443,16 +449,35 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
     return g;
 }
 
-// The gradient of each lane's region (waterfall over the distinct regions of active lanes).
+// The gradient of each lane's region.  Regions of one sphere or box node without rotation are
+// done for every lane at once, their node read per lane; the others in a waterfall over the
+// distinct regions of the remaining lanes.
 template <uint32_t KM>
 SMCRT_DEV V3<float> lane_region_gradient(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane,
                                          const V3<float>& p, const float* __restrict__ accel) {
     V3<float> res{0.f, 0.f, 0.f};
-    uint64_t pending = wave_ballot(active);
+    bool done = false;
+    if constexpr (KM_HAS(KM, KM_PRIM(SMCRT_SPHERE)) || KM_HAS(KM, KM_PRIM(SMCRT_BOX))) {
+        if (active) {
+            const DevRegion R = ((const DevRegion*)regions)[r_lane];
+            if (R.count == 1) {
+                const DevNode* n = (const DevNode*)nodes + R.first;
+                const int kind = n->kind, ident = n->ident;
+                const bool sphere = KM_HAS(KM, KM_PRIM(SMCRT_SPHERE)) && kind == SMCRT_SPHERE;
+                const bool box = KM_HAS(KM, KM_PRIM(SMCRT_BOX)) && kind == SMCRT_BOX;
+                if ((sphere || box) && n->op == SMCRT_OP_PRIM && ident != 0) {
+                    const V3<float> q = ident == 2 ? V3<float>{p.x + n->t[9], p.y + n->t[10], p.z + n->t[11]} : p;
+                    res = closed_form_gradient(sphere, n->p[0], n->p[1], n->p[2], q);
+                    done = true;
+                }
+            }
+        }
+    }
+    uint64_t pending = wave_ballot(active && !done);
     while (pending) {
         const int leader = __builtin_ctzll(pending);
         const int r = __builtin_amdgcn_readlane(r_lane, leader);
-        const bool mine = active && r_lane == r;
+        const bool mine = active && !done && r_lane == r;
         const int first = __builtin_amdgcn_readfirstlane(regions[r].first);
         const int count = __builtin_amdgcn_readfirstlane(regions[r].count);
         const int flat = __builtin_amdgcn_readfirstlane(regions[r].flat);

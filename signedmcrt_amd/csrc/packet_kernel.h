@@ -32,7 +32,10 @@
 
 namespace smcrt {
 
-enum : int { ST_IDLE = 0, ST_EMIT = 1, ST_START = 2, ST_MARCHED = 3, ST_CROSSED = 4 };
+// ST_SCATTER: the flight ended inside the grid; the interaction (albedo test, scatter) runs in
+// the next iteration, after the sweep at the interaction point, with the iteration's one event
+// block (shared by emission, Fresnel and interaction lanes: one Philox site per iteration).
+enum : int { ST_IDLE = 0, ST_EMIT = 1, ST_START = 2, ST_MARCHED = 3, ST_CROSSED = 4, ST_SCATTER = 5 };
 enum : int { END_ABSORB = 0, END_R = 1, END_T = 2, END_SURFACE = 3, END_BOUNCE = 4, END_ERR = 5 };
 
 // stats layout (unsigned long long): per set [NOUT], then global event counters [NEV]
@@ -178,12 +181,19 @@ SMCRT_DEV void trace_event(CKCold* C, Lane& L, int kind) {
     }
 }
 
-// emitters (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338.
-// The emission block: u0, u1 the emitter's draws, u2 the first flight's optical depth.
+// Whether the emitter's position depends on its draws (uniform, circular): those draw the
+// emission block at emission, before the sweep; the others (pencil, point) start at a fixed
+// position and draw it with the iteration's event block after the sweep.
 template <int KSRC>
-SMCRT_DEV void emit(const KParams& P, CKCold* C, Lane& L) {
-    float u[4];
-    draw_block(L, P, u);
+SMCRT_DEV bool emit_draws_position(CKCold* C) {
+    const int k = KSRC >= 0 ? KSRC : C->src_kind;
+    return k == 2 || k == 3;
+}
+
+// emitters (src/photon.f90): pencil :464-505, uniform :397-461, circular :214-287, point :290-338.
+// The emission block u: u0, u1 the emitter's draws, u2 the first flight's optical depth.
+template <int KSRC>
+SMCRT_DEV void emit(CKCold* C, Lane& L, const float u[4]) {
     L.tau = f_neglog(u[2]);
     switch (KSRC >= 0 ? KSRC : C->src_kind) {
     case 2: {  // uniform: draws rx, ry
@@ -786,12 +796,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
 
         const DevMat* mt = s_mat + L.set * P.nreg;
         double* __restrict__ grid = P.jmean ? P.jmean + (size_t)L.set * P.grid_stride : nullptr;
+        const uint32_t gbase = (uint32_t)L.set * (uint32_t)P.grid_stride;
 
         // ---- emission ----
-        if constexpr (kWaveOut) out[O_LAUNCHED] += __builtin_popcountll(wave_ballot(L.st == ST_EMIT));
-        if (L.st == ST_EMIT) {
+        const bool emitting = L.st == ST_EMIT;
+        if constexpr (kWaveOut) out[O_LAUNCHED] += __builtin_popcountll(wave_ballot(emitting));
+        const bool emit_now = emit_draws_position<SW::kSource>(C);  // wave-uniform
+        if (emitting) {
             rng_init(L.rng);
-            emit<SW::kSource>(P, C, L);
+            if (emit_now) {
+                float u[4];
+                draw_block(L, P, u);
+                emit<SW::kSource>(C, L, u);
+            } else {  // fixed start: the draws follow with the iteration's event block
+                L.px = C->spos[0]; L.py = C->spos[1]; L.pz = C->spos[2];
+            }
             L.rx = 0.f; L.ry = 0.f; L.rz = 0.f;
             L.bounces = 0;
             L.nsc = 0;
@@ -950,45 +969,121 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         if (wave_ballot(want_normal))
             gxyz = lane_region_gradient<KM>(nodes, regions, want_normal, use_region, V3<float>{L.px, L.py, L.pz}, P.accel);
 
-        // normal + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
-        if (const uint64_t m = wave_ballot(want_normal)) {
-            ev[E_NORMALS] += __builtin_popcountll(m);
-            if (want_normal) {
-                float inv = f_rsqrt(gxyz.x * gxyz.x + gxyz.y * gxyz.y + gxyz.z * gxyz.z);
-                float nx = gxyz.x * inv, ny = gxyz.y * inv, nz = gxyz.z * inv;
-                const int nl = fn;
-                const float n1 = mt[L.layer].n, n2 = mt[nl].n;
-                float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);
-                float u[4];
-                draw_block(L, P, u);  // the Fresnel event's block
-                bool rflag = u[0] <= R;  // reflect or refract
-                if constexpr (kWeight) L.weight *= R;  // on either branch (src/inttau2.f90:167)
-                if (rflag) {
-                    float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
-                    L.dx -= c * nx; L.dy -= c * ny; L.dz -= c * nz;
-                } else {
-                    float eta = f_div(n1, n2);
-                    float c1 = nx * L.dx + ny * L.dy + nz * L.dz;
-                    if (c1 < 0.f) c1 = -c1;
-                    else { nx = -nx; ny = -ny; nz = -nz; }
-                    float c2 = f_sqrt(fmaxf(1.f - eta * eta * (1.f - c1 * c1), 0.f));
-                    float k = eta * c1 - c2;
-                    L.dx = eta * L.dx + k * nx; L.dy = eta * L.dy + k * ny; L.dz = eta * L.dz + k * nz;
+        // ---- the iteration's event block: emission (fixed-start emitters), Fresnel, interaction ----
+        const bool emit_late = emitting && !emit_now && st0 == ST_START;
+        const bool interact = st0 == ST_SCATTER;
+        const bool blk = emit_late || want_normal || interact;
+        bool scat = false;   // this lane's packet scattered
+        bool wpush = false;  // weight_scatter deposit for the global tally
+        uint32_t widx = 0u;
+        float wlen = 0.f;
+        if (wave_ballot(blk)) {
+            float u[4] = {0.f, 0.f, 0.f, 0.f};
+            if (blk) draw_block(L, P, u);
+            if (emit_late) emit<SW::kSource>(C, L, u);
+            // normal + Fresnel (src/inttau2.f90:163-181, src/surfaces.f90:14-84)
+            if (const uint64_t m = wave_ballot(want_normal)) {
+                ev[E_NORMALS] += __builtin_popcountll(m);
+                if (want_normal) {
+                    float inv = f_rsqrt(gxyz.x * gxyz.x + gxyz.y * gxyz.y + gxyz.z * gxyz.z);
+                    float nx = gxyz.x * inv, ny = gxyz.y * inv, nz = gxyz.z * inv;
+                    const int nl = fn;
+                    const float n1 = mt[L.layer].n, n2 = mt[nl].n;
+                    float R = fresnel(L.dx, L.dy, L.dz, nx, ny, nz, n1, n2);
+                    bool rflag = u[0] <= R;  // reflect or refract
+                    if constexpr (kWeight) L.weight *= R;  // on either branch (src/inttau2.f90:167)
+                    if (rflag) {
+                        float c = 2.f * (nx * L.dx + ny * L.dy + nz * L.dz);
+                        L.dx -= c * nx; L.dy -= c * ny; L.dz -= c * nz;
+                    } else {
+                        float eta = f_div(n1, n2);
+                        float c1 = nx * L.dx + ny * L.dy + nz * L.dz;
+                        if (c1 < 0.f) c1 = -c1;
+                        else { nx = -nx; ny = -ny; nz = -nz; }
+                        float c2 = f_sqrt(fmaxf(1.f - eta * eta * (1.f - c1 * c1), 0.f));
+                        float k = eta * c1 - c2;
+                        L.dx = eta * L.dx + k * nx; L.dy = eta * L.dy + k * ny; L.dz = eta * L.dz + k * nz;
+                    }
+                    L.tau = f_neglog(u[1]);  // resampled optical depth (:168)
+                    L.taurun = 0.f;
+                    L.iface = true;
+                    if (!rflag) {
+                        L.layer = nl;
+                        trace_ev(3);
+                    } else {
+                        L.px = L.opx; L.py = L.opy; L.pz = L.opz;  // step back inside (:174)
+                        trace_ev(4);
+                        if (++L.bounces > C->max_bounces) end_kind = END_BOUNCE;
+                    }
+                    if (end_kind < 0) L.st = ST_START;
                 }
-                L.tau = f_neglog(u[1]);  // resampled optical depth (:168)
-                L.taurun = 0.f;
-                L.iface = true;
-                if (!rflag) {
-                    L.layer = nl;
-                    trace_ev(3);
-                } else {
-                    L.px = L.opx; L.py = L.opy; L.pz = L.opz;  // step back inside (:174)
-                    trace_ev(4);
-                    if (++L.bounces > C->max_bounces) end_kind = END_BOUNCE;
-                }
-                if (end_kind < 0) L.st = ST_START;
             }
+            // interaction at the end of a flight inside the grid
+            bool absorbed = false;  // ended here (the sweep at this point served nothing)
+            if (interact) {
+                const DevMat m = mt[L.layer];
+                if (kWeight && (C->flags & 16u)) {
+                    // weight_scatter (src/kernelsMod.f90:83-111): deposit the absorbed fraction in
+                    // the interaction voxel (leaving the grid there ends the packet), scatter, then
+                    // Russian roulette below THRESHOLD = 0.01 with CHANCE = 0.1 (src/constants.f90)
+                    const float wa = L.weight * (1.f - m.albedo);
+                    L.weight -= wa;
+                    const int vx = (int)floorf((L.px + P.xmax) * P.sx), vy = (int)floorf((L.py + P.ymax) * P.sy),
+                              vz = (int)floorf((L.pz + P.zmax) * P.sz);
+                    if ((unsigned)vx >= (unsigned)P.nx || (unsigned)vy >= (unsigned)P.ny || (unsigned)vz >= (unsigned)P.nz) {
+                        end_kind = classify_escape(C, L);
+                    } else {
+                        if (grid) {
+                            wpush = true;
+                            widx = gbase + (uint32_t)(vx + P.nx * (vy + P.ny * vz));
+                            wlen = wa;
+                        }
+                        L.nsc++;
+                        scat = true;
+                        scatter(L, m.hgg, m.g2, u[1], u[2]);
+                        trace_ev(2);
+                        if (L.weight < 0.01f) {
+                            if (u[0] < 0.1f) L.weight *= 10.f;  // roulette
+                            else end_kind = END_ABSORB;
+                        }
+                    }
+                } else if (u[0] < m.albedo) {
+                    // albedo test (src/kernelsMod.f90:287-297)
+                    if (ADJ && m.mat >= 0) {
+                        adj_sc(A, 2, m.mat, P.nmat) += 1.f;
+                        adj_sc(A, 3, m.mat, P.nmat) += adjC;
+                    }
+                    L.nsc++;
+                    scat = true;
+                    if (kDiag && (C->flags & 1u) && L.nsc <= 4) {
+                        double* mo = C->moments + (L.nsc - 1) * 3;
+                        atomicAdd(mo + 0, (double)L.px); atomicAdd(mo + 1, (double)L.py); atomicAdd(mo + 2, (double)L.pz);
+                        atomicAdd(mo + 12, (double)L.px * L.px); atomicAdd(mo + 13, (double)L.py * L.py);
+                        atomicAdd(mo + 14, (double)L.pz * L.pz);
+                    }
+                    scatter(L, m.hgg, m.g2, u[1], u[2]);
+                    trace_ev(2);
+                    if (kDiag && (C->flags & 2u) && L.nsc >= 5) end_kind = END_ABSORB;
+                } else {
+                    end_kind = END_ABSORB;
+                }
+                absorbed = end_kind >= 0;
+                if (!absorbed) {
+                    // the next flight starts here: its optical depth, then the flight-start test
+                    // against the sweep already made at this point (src/inttau2.f90:58-61)
+                    L.tau = f_neglog(u[3]);
+                    L.taurun = 0.f;
+                    L.iface = false;
+                    L.st = ST_START;
+                    if (dmin < eps_kill) end_kind = END_SURFACE;
+                    else do_march = true;
+                }
+            }
+            ev[E_SWEEPS] -= __builtin_popcountll(wave_ballot(absorbed));
         }
+        if constexpr (kWaveOut) out[O_SCATTERS] += __builtin_popcountll(wave_ballot(scat));
+        else out[O_SCATTERS] += scat ? 1u : 0u;
+        if (kWeight && P.jmean) ev[E_ATOMICS] += tally_push(Q, wpush, widx, wlen, tbase, lane_id);
 
         // ---- march step (src/inttau2.f90:64-83): optical depth, then the voxel walk ----
         ev[E_STEPS] += __builtin_popcountll(wave_ballot(do_march));
@@ -1014,7 +1109,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         // (the loop is wave-uniform so the tally queue can be filled between lane steps)
         bool walk_done = W.active && !W.inside;  // started outside the grid
         bool walking = W.active && W.inside;
-        const uint32_t gbase = (uint32_t)L.set * (uint32_t)P.grid_stride;
         // per-axis step of the crossing distance (the direction is fixed over the walk)
         const float dtx = f_rcp(fabsf(L.dx) * P.sx), dty = f_rcp(fabsf(L.dy) * P.sy), dtz = f_rcp(fabsf(L.dz) * P.sz);
         const int sgx = L.dx > 0.f ? 1 : -1, sgy = L.dy > 0.f ? 1 : -1, sgz = L.dz > 0.f ? 1 : -1;
@@ -1077,11 +1171,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             if (P.jmean) ev[E_ATOMICS] += tally_push(Q, gpush, gidx, glen, tbase, lane_id);
         }
 
-        // ---- step finished: move, then end of flight (albedo test) or next march step ----
-        bool scat = false;   // this lane's packet scattered
-        bool wpush = false;  // weight_scatter deposit for the global tally
-        uint32_t widx = 0u;
-        float wlen = 0.f;
+        // ---- step finished: move, then end of flight (interaction next iteration) or next march step ----
         if (walk_done) {
             W.active = false;
             const float d = W.d;
@@ -1096,8 +1186,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 }
             }
             move(L, d);
-            float u[4];  // the flight end's block (one site for both estimators)
-            if (W.inside && W.flight_over) draw_block(L, P, u);
             if (!W.inside) {
                 // left the grid (src/inttau2.f90:256-259,282-285): the outcome is fixed.  With
                 // detectors the packet moves on, untallied, to the end of its flight or the next
@@ -1108,76 +1196,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 } else {
                     end_kind = classify_escape(C, L);
                 }
-            } else if (kWeight && W.flight_over && (C->flags & 16u)) {
-                // weight_scatter (src/kernelsMod.f90:83-111): deposit the absorbed fraction in the
-                // interaction voxel (leaving the grid there ends the packet), scatter, then
-                // Russian roulette below THRESHOLD = 0.01 with CHANCE = 0.1 (src/constants.f90)
-                const DevMat m = mt[L.layer];
-                const float wa = L.weight * (1.f - m.albedo);
-                L.weight -= wa;
-                const int vx = (int)floorf((L.px + P.xmax) * P.sx), vy = (int)floorf((L.py + P.ymax) * P.sy),
-                          vz = (int)floorf((L.pz + P.zmax) * P.sz);
-                if ((unsigned)vx >= (unsigned)P.nx || (unsigned)vy >= (unsigned)P.ny || (unsigned)vz >= (unsigned)P.nz) {
-                    end_kind = classify_escape(C, L);
-                } else {
-                    if (grid) {
-                        wpush = true;
-                        widx = gbase + (uint32_t)(vx + P.nx * (vy + P.ny * vz));
-                        wlen = wa;
-                    }
-                    L.nsc++;
-                    scat = true;
-                    scatter(L, m.hgg, m.g2, u[1], u[2]);
-                    trace_ev(2);
-                    bool alive = true;
-                    if (L.weight < 0.01f) {
-                        if (u[0] < 0.1f) L.weight *= 10.f;  // roulette
-                        else { end_kind = END_ABSORB; alive = false; }
-                    }
-                    if (alive) {
-                        L.tau = f_neglog(u[3]);  // next flight's optical depth
-                        L.taurun = 0.f;
-                        L.iface = false;
-                        L.st = ST_START;
-                    }
-                }
-            } else if (W.flight_over) {
-                // albedo test (src/kernelsMod.f90:287-297)
-                const DevMat m = mt[L.layer];
-                if (u[0] < m.albedo) {
-                    if (ADJ && m.mat >= 0) {
-                        adj_sc(A, 2, m.mat, P.nmat) += 1.f;
-                        adj_sc(A, 3, m.mat, P.nmat) += adjC;
-                    }
-                    L.nsc++;
-                    scat = true;
-                    if (kDiag && (C->flags & 1u) && L.nsc <= 4) {
-                        double* mo = C->moments + (L.nsc - 1) * 3;
-                        atomicAdd(mo + 0, (double)L.px); atomicAdd(mo + 1, (double)L.py); atomicAdd(mo + 2, (double)L.pz);
-                        atomicAdd(mo + 12, (double)L.px * L.px); atomicAdd(mo + 13, (double)L.py * L.py);
-                        atomicAdd(mo + 14, (double)L.pz * L.pz);
-                    }
-                    scatter(L, m.hgg, m.g2, u[1], u[2]);
-                    trace_ev(2);
-                    if (kDiag && (C->flags & 2u) && L.nsc >= 5) {
-                        end_kind = END_ABSORB;
-                    } else {
-                        L.tau = f_neglog(u[3]);  // next flight's optical depth
-                        L.taurun = 0.f;
-                        L.iface = false;
-                        L.st = ST_START;
-                    }
-                } else {
-                    end_kind = END_ABSORB;
-                }
             } else {
-                L.st = ST_MARCHED;
+                L.st = W.flight_over ? ST_SCATTER : ST_MARCHED;
             }
         }
-
-        if constexpr (kWaveOut) out[O_SCATTERS] += __builtin_popcountll(wave_ballot(scat));
-        else out[O_SCATTERS] += scat ? 1u : 0u;
-        if (P.jmean) ev[E_ATOMICS] += tally_push(Q, wpush, widx, wlen, tbase, lane_id);
 
         // non-finite guard
         const bool nonfinite = end_kind < 0 && L.st != ST_IDLE && !W.active &&

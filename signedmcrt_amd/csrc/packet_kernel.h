@@ -48,6 +48,8 @@ enum : int {
 
 // most region-indexed material entries staged in LDS (nsets * nregions)
 #define SMCRT_LDS_MATS 2048
+// most regions with a gradient descriptor in LDS (later regions take the waterfall)
+#define SMCRT_LDS_GRAD 128
 
 // Parameters used off the hot path (emission, outcome classification, tracing, moments, the
 // adjoint), kept in device memory and read with scalar loads where they are used so that they
@@ -459,27 +461,46 @@ SMCRT_DEV V3<float> region_gradient(CDevNode* nodes, int first, int count, int f
     return g;
 }
 
-// The gradient of each lane's region.  Regions of one sphere or box node without rotation are
-// done for every lane at once, their node read per lane; the others in a waterfall over the
-// distinct regions of the remaining lanes.
+// A region's closed-form gradient, when it is one sphere or box node without rotation: kind 1
+// sphere, 2 box, 0 none; the node's translation (zero for the identity) and half sizes.  Staged
+// in LDS per block so the per-lane reads wait on LDS, not on the wave's outstanding atomics.
+struct alignas(16) GradDesc {
+    int kind;
+    float tx, ty, tz, hx, hy, hz, pad;
+};
+
+__host__ __device__ inline int grad_desc_count(int nreg) { return nreg < SMCRT_LDS_GRAD ? nreg : SMCRT_LDS_GRAD; }
+
 template <uint32_t KM>
-SMCRT_DEV V3<float> lane_region_gradient(CDevNode* nodes, CDevRegion* regions, bool active, int r_lane,
-                                         const V3<float>& p, const float* __restrict__ accel) {
+SMCRT_DEV GradDesc grad_desc(CDevNode* nodes, CDevRegion* regions, int r) {
+    GradDesc g{0, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const DevRegion R = ((const DevRegion*)regions)[r];
+    if (R.count != 1) return g;
+    const DevNode* n = (const DevNode*)nodes + R.first;
+    const bool sphere = KM_HAS(KM, KM_PRIM(SMCRT_SPHERE)) && n->kind == SMCRT_SPHERE;
+    const bool box = KM_HAS(KM, KM_PRIM(SMCRT_BOX)) && n->kind == SMCRT_BOX;
+    if (!(sphere || box) || n->op != SMCRT_OP_PRIM || n->ident == 0) return g;
+    g.kind = sphere ? 1 : 2;
+    if (n->ident == 2) { g.tx = n->t[9]; g.ty = n->t[10]; g.tz = n->t[11]; }
+    g.hx = n->p[0]; g.hy = n->p[1]; g.hz = n->p[2];
+    return g;
+}
+
+// The gradient of each lane's region.  Regions with a descriptor (one sphere or box node without
+// rotation) are done for every lane at once; the others in a waterfall over the distinct regions
+// of the remaining lanes.
+template <uint32_t KM>
+SMCRT_DEV V3<float> lane_region_gradient(CDevNode* nodes, CDevRegion* regions, const GradDesc* gd, int n_gd,
+                                         bool active, int r_lane, const V3<float>& p, const float* __restrict__ accel) {
     V3<float> res{0.f, 0.f, 0.f};
     bool done = false;
     if constexpr (KM_HAS(KM, KM_PRIM(SMCRT_SPHERE)) || KM_HAS(KM, KM_PRIM(SMCRT_BOX))) {
-        if (active) {
-            const DevRegion R = ((const DevRegion*)regions)[r_lane];
-            if (R.count == 1) {
-                const DevNode* n = (const DevNode*)nodes + R.first;
-                const int kind = n->kind, ident = n->ident;
-                const bool sphere = KM_HAS(KM, KM_PRIM(SMCRT_SPHERE)) && kind == SMCRT_SPHERE;
-                const bool box = KM_HAS(KM, KM_PRIM(SMCRT_BOX)) && kind == SMCRT_BOX;
-                if ((sphere || box) && n->op == SMCRT_OP_PRIM && ident != 0) {
-                    const V3<float> q = ident == 2 ? V3<float>{p.x + n->t[9], p.y + n->t[10], p.z + n->t[11]} : p;
-                    res = closed_form_gradient(sphere, n->p[0], n->p[1], n->p[2], q);
-                    done = true;
-                }
+        if (active && r_lane < n_gd) {
+            const GradDesc g = gd[r_lane];
+            if (g.kind != 0) {
+                const V3<float> q{p.x + g.tx, p.y + g.ty, p.z + g.tz};
+                res = closed_form_gradient(g.kind == 1, g.hx, g.hy, g.hz, q);
+                done = true;
             }
         }
     }
@@ -676,7 +697,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     for (int i = threadIdx.x; i < n_mats; i += blockDim.x) s_mat[i] = P.mats[i];
     // the tally tile follows the materials (FP64 sums: the LDS adds round like FP32 register adds,
     // and an FP32 sum of a block's ~1e4-1e5 chords into a hot voxel drifts by ~1e-5 relative)
-    double* const s_tile = reinterpret_cast<double*>(smem + sizeof(DevMat) * n_mats);
+    // the regions' gradient descriptors follow the materials
+    GradDesc* const s_gd = reinterpret_cast<GradDesc*>(smem + sizeof(DevMat) * n_mats);
+    const int n_gd = grad_desc_count(P.nreg);
+    for (int r = threadIdx.x; r < n_gd; r += blockDim.x) s_gd[r] = grad_desc<KM>(nodes, regions, r);
+    const size_t lds_head = sizeof(DevMat) * n_mats + sizeof(GradDesc) * n_gd;
+    double* const s_tile = reinterpret_cast<double*>(smem + lds_head);
     const int tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) s_tile[i] = 0.0;
     // this block's tally target (its XCD's partial grid; workgroups go to XCDs round-robin)
@@ -685,14 +711,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     TallyQueue Q;
     Q.n = 0;
     {
-        size_t qoff = sizeof(DevMat) * n_mats + sizeof(double) * ((tile_vox + 1) & ~1);
+        size_t qoff = lds_head + sizeof(double) * ((tile_vox + 1) & ~1);
         if (ADJ) qoff += sizeof(float) * 4 * P.nmat * blockDim.x + sizeof(double) * 4 * P.nmat;
         Q.e = reinterpret_cast<uint2*>(smem + qoff) + (threadIdx.x / 64) * TALLY_QUEUE;
     }
     AdjLds A;
     if (ADJ) {
         A.stride = blockDim.x;
-        unsigned char* base = smem + sizeof(DevMat) * n_mats + sizeof(double) * ((tile_vox + 1) & ~1);
+        unsigned char* base = smem + lds_head + sizeof(double) * ((tile_vox + 1) & ~1);
         A.sc = reinterpret_cast<float*>(base);
         A.g = reinterpret_cast<double*>(base + sizeof(float) * 4 * P.nmat * blockDim.x);
         for (int i = threadIdx.x; i < 4 * P.nmat; i += blockDim.x) A.g[i] = 0.0;
@@ -967,7 +993,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         // exact gradient of the normal's region (one forward-mode pass; flat folds lazily)
         V3<float> gxyz{0.f, 0.f, 0.f};
         if (wave_ballot(want_normal))
-            gxyz = lane_region_gradient<KM>(nodes, regions, want_normal, use_region, V3<float>{L.px, L.py, L.pz}, P.accel);
+            gxyz = lane_region_gradient<KM>(nodes, regions, s_gd, n_gd, want_normal, use_region, V3<float>{L.px, L.py, L.pz},
+                                            P.accel);
 
         // ---- the iteration's event block: emission (fixed-start emitters), Fresnel, interaction ----
         const bool emit_late = emitting && !emit_now && st0 == ST_START;

@@ -1253,7 +1253,9 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     L.walk_cap = (o && o->walk_cap > 0) ? o->walk_cap : 4;
     L.refill_min = (o && o->refill_min > 0) ? (o->refill_min > 64 ? 64 : o->refill_min) : 1;
     L.waves = compiled ? compiled_waves(o) : launch_waves(s, o, adj);
-    const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) + sizeof(double) * ((tile_vox + 1) & ~1);
+    // LDS: material tables, gradient descriptors, the tally tile, then the adjoint's and the queues
+    const size_t mats = (size_t)n_sets * s->regions.size() * sizeof(DevMat) +
+                        sizeof(GradDesc) * grad_desc_count((int)s->regions.size()) + sizeof(double) * ((tile_vox + 1) & ~1);
     L.block = (L.block / 64) * 64;
     if (L.block < 64) L.block = 64;
     if (L.block > 256) L.block = 256;
@@ -1352,7 +1354,8 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
                               hipSuccess ||
                           bpc < 1))
                 bpc = w;
-            const long mats = (long)s->regions.size() * (long)sizeof(DevMat);
+            const long mats = (long)s->regions.size() * (long)sizeof(DevMat) +
+                              (long)sizeof(GradDesc) * grad_desc_count((int)s->regions.size());
             const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
             const long per_block = (long)lds_cu / bpc - mats - queue - 256;
             const int budget = (int)std::max(0L, std::min(8192L, per_block / (long)sizeof(double)));

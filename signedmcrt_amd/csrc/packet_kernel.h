@@ -1139,6 +1139,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         // per-axis step of the crossing distance (the direction is fixed over the walk)
         const float dtx = f_rcp(fabsf(L.dx) * P.sx), dty = f_rcp(fabsf(L.dy) * P.sy), dtz = f_rcp(fabsf(L.dz) * P.sz);
         const int sgx = L.dx > 0.f ? 1 : -1, sgy = L.dy > 0.f ? 1 : -1, sgz = L.dz > 0.f ? 1 : -1;
+        // flat-index step per axis (x: sgx)
+        const int ystep = L.dy > 0.f ? P.nx : -P.nx, zstep = L.dz > 0.f ? P.nx * P.ny : -(P.nx * P.ny);
         // one chord per lane and trip, written as selects rather than nested branches (lanes of a
         // wavefront differ in which axis they step and whether they merge, flush or leave)
         for (int k = 0; k < cap; ++k) {
@@ -1168,29 +1170,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     glen = pend_len;
                     const unsigned lx = (unsigned)(W.ix - P.t0[0]), ly = (unsigned)(W.iy - P.t0[1]),
                                    lz = (unsigned)(W.iz - P.t0[2]);
-                    const int li = (lx < (unsigned)P.tn[0] && ly < (unsigned)P.tn[1] && lz < (unsigned)P.tn[2])
-                                       ? (int)(lx + P.tn[0] * (ly + P.tn[1] * lz)) : -1;
+                    // (bitwise ands: no short-circuit branches; 24-bit products: the tile holds at
+                    // most 8192 voxels)
+                    const bool in_tile = (lx < (unsigned)P.tn[0]) & (ly < (unsigned)P.tn[1]) & (lz < (unsigned)P.tn[2]);
+                    const int li = in_tile ? (int)(lx + __umul24((unsigned)P.tn[0], ly + __umul24((unsigned)P.tn[1], lz))) : -1;
                     pend_len = flush ? len : (crossed ? pend_len + len : pend_len);
                     pend_li = flush ? li : pend_li;
                     pend_idx = flush ? W.idx : pend_idx;
                 }
-                // next voxel boundary: the axis with the nearest crossing (x, then y, on ties)
-                const bool ax = W.tmx <= W.tmy && W.tmx <= W.tmz;
-                const bool ay = !ax && W.tmy <= W.tmz;
-                const int c = ax ? W.ix + sgx : (ay ? W.iy + sgy : W.iz + sgz);
-                const int lim = ax ? P.nx : (ay ? P.ny : P.nz);
-                const bool ends = tn >= W.d;                  // the step ends inside this voxel
-                const bool leaves = !ends && (unsigned)c >= (unsigned)lim;  // into the grid's outside
-                const bool steps = !ends && !leaves;
+                // next voxel boundary: the axis with the nearest crossing (x, then y, on ties); the
+                // candidate indices of all three axes and selects, no branches
+                const bool ax = (W.tmx <= W.tmy) & (W.tmx <= W.tmz);
+                const bool ay = !ax & (W.tmy <= W.tmz);
+                const bool az = !ax & !ay;
+                const int jx = W.ix + (ax ? sgx : 0), jy = W.iy + (ay ? sgy : 0), jz = W.iz + (az ? sgz : 0);
+                const bool out = ((unsigned)jx >= (unsigned)P.nx) | ((unsigned)jy >= (unsigned)P.ny) |
+                                 ((unsigned)jz >= (unsigned)P.nz);
+                const bool ends = tn >= W.d;     // the step ends inside this voxel
+                const bool leaves = !ends & out;  // into the grid's outside
+                const bool steps = !ends & !out;
                 W.t = ends ? W.t : tn;
                 W.inside = !leaves;
-                W.ix = (steps && ax) ? c : W.ix;
-                W.iy = (steps && ay) ? c : W.iy;
-                W.iz = (steps && !ax && !ay) ? c : W.iz;
-                W.idx += steps ? (ax ? sgx : (ay ? sgy * P.nx : sgz * P.nx * P.ny)) : 0;
-                W.tmx = (steps && ax) ? W.tmx + dtx : W.tmx;
-                W.tmy = (steps && ay) ? W.tmy + dty : W.tmy;
-                W.tmz = (steps && !ax && !ay) ? W.tmz + dtz : W.tmz;
+                W.ix = steps ? jx : W.ix;
+                W.iy = steps ? jy : W.iy;
+                W.iz = steps ? jz : W.iz;
+                W.idx += steps ? (ax ? sgx : (ay ? ystep : zstep)) : 0;
+                W.tmx = (steps & ax) ? W.tmx + dtx : W.tmx;
+                W.tmy = (steps & ay) ? W.tmy + dty : W.tmy;
+                W.tmz = (steps & az) ? W.tmz + dtz : W.tmz;
                 walk_done = !steps;
                 walking = steps;
             }

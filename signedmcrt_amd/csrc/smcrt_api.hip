@@ -187,7 +187,8 @@ struct smcrt_scene {
     std::vector<smcrt_detector> dets;
     std::vector<DevDet> ddets;
     size_t det_bins = 0;
-    // LDS tally tile (voxel box) for the 2- and 3-wave kernels, chosen from a pilot run on first use
+    // LDS tally tile (voxel box) for register occupancies of 2, 3 and 4 blocks per CU, chosen from a
+    // pilot run on first use
     bool tile_ready = false;
     int tile_t0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, tile_tn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     // device
@@ -1231,6 +1232,14 @@ static hipError_t kernel_occupancy(const smcrt_scene* s, bool adj, int waves, bo
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(active, packet_kernel_for(s, adj, waves), block, lds);
 }
 
+// 256-lane blocks per CU the kernel's registers allow, clamped to 2..4: the index + 2 of the tally
+// tile sized for that occupancy (the 2-wave build of a small scene program can fit 4 waves).
+static int register_blocks(const smcrt_scene* s, bool adj, int waves, bool compiled) {
+    int active = 0;
+    if (kernel_occupancy(s, adj, waves, compiled, 256, 0, &active) != hipSuccess || active < 2) return 2;
+    return active > 4 ? 4 : active;
+}
+
 static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const LaunchShape& sh, hipStream_t stream,
                                        KParams& P) {
     void* args[] = {&P};
@@ -1342,25 +1351,20 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
     int rc = launch_run(s, &po, pilot, false, nullptr, stream, &ms);
     std::vector<double> h(nv);
     if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
-        // the tile takes the LDS left per 256-lane block at the kernel's register occupancy
-        // (2, 3 or 4 blocks per CU for the 2-, 3- and 4-wave builds), at most 64 KiB of FP64 sums
+        // one tile per register occupancy of 2, 3 or 4 256-lane blocks per CU (a launch takes the
+        // one its kernel's registers allow): the LDS left per block, at most 64 KiB of FP64 sums
         int lds_cu = 0;
         if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
             lds_cu <= 0)
             lds_cu = 160 * 1024;
-        for (int w = 2; w <= 4; ++w) {
-            int bpc = w;  // the 4-wave build exists only compiled at run time: 4 blocks per CU
-            if (w < 4 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, packet_kernel_for(s, false, w), 256, 0) !=
-                              hipSuccess ||
-                          bpc < 1))
-                bpc = w;
+        for (int bpc = 2; bpc <= 4; ++bpc) {
             const long mats = (long)s->regions.size() * (long)sizeof(DevMat) +
                               (long)sizeof(GradDesc) * grad_desc_count((int)s->regions.size());
             const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
             const long per_block = (long)lds_cu / bpc - mats - queue - 256;
             const int budget = (int)std::max(0L, std::min(8192L, per_block / (long)sizeof(double)));
             if (budget > 0)
-                choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0[w - 2], s->tile_tn[w - 2]);
+                choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0[bpc - 2], s->tile_tn[bpc - 2]);
         }
     }
     (void)hipFree(pilot);
@@ -1429,7 +1433,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     const bool compiled = jit_wanted(o) && jit_build(s);
     int tile_vox = 0;
     if (use_tile) {
-        const int wi = (compiled ? compiled_waves(o) : launch_waves(s, o, adj)) - 2;
+        const int wi = register_blocks(s, adj, compiled ? compiled_waves(o) : launch_waves(s, o, adj), compiled) - 2;
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }
@@ -2085,7 +2089,7 @@ extern "C" int smcrt_run_device_noalloc(smcrt_scene* s, const smcrt_run_opts* o,
     const bool use_tile = d_jmean && !(o->flags & SMCRT_FLAG_NO_LDS_TILE) && s->tile_ready;
     int tile_vox = 0;
     if (use_tile) {
-        const int wi = (compiled ? compiled_waves(o) : launch_waves(s, o, false)) - 2;
+        const int wi = register_blocks(s, false, compiled ? compiled_waves(o) : launch_waves(s, o, false), compiled) - 2;
         for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }

@@ -186,10 +186,12 @@ def main():
         tuned = {"shape": eng.shape, "given": True}
     elif not args.no_tune:
         t_tune = time.perf_counter()
-        res = eng.tune(use_cache=False)
+        res = eng.tune(use_cache=False, adjoint=bool(grad_spec and args.grad_steps > 0))
         tuned = {"shape": res.shape, "probe_kernel_ms": res.kernel_ms,
                  "default_probe_kernel_ms": res.default_ms, "speedup_vs_default": res.speedup,
                  "points": len(res.table), "seconds": time.perf_counter() - t_tune}
+        if res.adjoint_ms:
+            tuned["adjoint_probe_kernel_ms"] = {str(w): t for w, t in sorted(res.adjoint_ms.items())}
 
     for w in range(args.warmup):
         eng.run_device(n, d_grid.data_ptr(), seed=1000 + w, first_packet=0)
@@ -390,7 +392,7 @@ def bench_batched(args):
     shape = {}
     tuned = None
     if not args.no_tune:
-        res = engines[1].tune(use_cache=False)
+        res = engines[1].tune(use_cache=False, adjoint=False)
         shape = {k: v for k, v in res.shape.items() if k != "tile"}  # batches tally straight to HBM
         tuned = {"shape": res.shape, "tuned_on": "refracting_sphere 64^3", "speedup_vs_default": res.speedup}
     b = Batch(engines, sets, seed=123456789, **shape)

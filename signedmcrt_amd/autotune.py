@@ -121,6 +121,7 @@ class TuneResult:
     default_ms: float          # same at the library defaults
     table: List[Tuple[Dict, float]] = field(default_factory=list)
     cached: bool = False
+    adjoint_ms: Dict[int, float] = field(default_factory=dict)  # adjoint probe time per waves
 
     @property
     def speedup(self) -> float:
@@ -136,7 +137,7 @@ def _device_name() -> str:
 def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
          cache_path: Optional[str] = None, use_cache: bool = True,
          space: Optional[Dict[str, Sequence]] = None, seed: int = 7919,
-         target_ms: float = 150.0, min_gain: float = 0.02) -> TuneResult:
+         target_ms: float = 150.0, min_gain: float = 0.02, adjoint: bool = True) -> TuneResult:
     """Tune the launch shape of `engine`'s scene on the current GPU.
 
     n_probe: packets per probe (default: enough for `target_ms` of kernel time at the default
@@ -144,7 +145,9 @@ def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
     few milliseconds of every launch most lanes are idle, which favours small blocks and chunks
     in short probes but not in long runs, so the probes are long enough for the steady state to
     dominate.  A knob moves away from the library default only for a gain above `min_gain`.
-    The probe tallies into a scratch device grid."""
+    The probe tallies into a scratch device grid.  With `adjoint`, the adjoint kernel's resident
+    waves are tuned as well, on the forward's best shape (its register budget differs: it keeps
+    per-material scores), and returned as the shape's "adj_waves" (used by Engine.grad)."""
     import torch
 
     space = dict(space or SPACE)
@@ -192,8 +195,27 @@ def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
     t0 = time.perf_counter()
     best, table = coordinate_descent(lambda sh: probe(sh, n_probe), space, start, sweeps, min_gain)
     best_ms = dict((tuple(sorted(s.items())), c) for s, c in table)[tuple(sorted(best.items()))]
+    if adjoint and "waves" in space:
+        # same packets at every point: the counters must agree across the adjoint's probes
+        adj: Dict[int, float] = {}
+        adj_ref: Dict = {}
+        for w in space["waves"]:
+            kw = run_kwargs(dict(best, waves=w))
+            kw.pop("flags")  # the LDS tile option: the adjoint does not tally
+            t = float("inf")
+            for _ in range(reps):
+                st = engine.grad(n_probe, seed=seed, first_packet=1 << 50, **kw).stats
+                inv = {k: st[k] for k in INVARIANT}
+                if adj_ref and adj_ref != inv:
+                    raise RuntimeError(f"adjoint at {w} waves changed the transported work: {inv} != {adj_ref}")
+                adj_ref = adj_ref or inv
+                t = min(t, st["kernel_ms"])
+            adj[w] = t
+        fastest = min(adj, key=adj.get)
+        aw = fastest if adj[fastest] < adj[best["waves"]] * (1.0 - min_gain) else best["waves"]
+        best = dict(best, adj_waves=int(aw))
     default_ms = table[0][1]
-    res = TuneResult(best, best_ms, default_ms, table)
+    res = TuneResult(best, best_ms, default_ms, table, adjoint_ms=adj if adjoint and "waves" in space else {})
     if use_cache:
         try:
             os.makedirs(os.path.dirname(cache_path), exist_ok=True)

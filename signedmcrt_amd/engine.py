@@ -186,6 +186,9 @@ class Engine:
         at most 16 — the others' derivatives come back as 0)."""
         g = self.scene.grid
         nm = self.scene.n_materials
+        # the adjoint kernel's own tuned resident waves, unless given
+        if "waves" not in shape and self.shape.get("adj_waves"):
+            shape = dict(shape, waves=int(self.shape["adj_waves"]))
         flags, shape = self._shape(0, shape)
         o = run_opts(n, seed, first_packet, flags, **shape)
         if materials is not None:

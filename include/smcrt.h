@@ -193,10 +193,19 @@ enum {
                                        always scatter, Russian roulette below w = 0.01 (survival
                                        0.1, weight x10); Fresnel events scale w by R; detectors
                                        bin the final weight.  Roulette kills count as absorbed. */
-    SMCRT_FLAG_SHARED_GRID = 1u << 5 /* tally every block into the one grid (default: one FP64
+    SMCRT_FLAG_SHARED_GRID = 1u << 5, /* tally every block into the one grid (default: one FP64
                                        partial grid per XCD, summed after the launch; same
                                        result up to FP64 summation order, ~2x faster where
                                        tally atomics dominate; costs n_xcd x grid memory)      */
+    SMCRT_FLAG_DIRECT_TALLY = 1u << 6, /* chords outside the LDS tile go to FP64 atomics      */
+    SMCRT_FLAG_BINNED_TALLY = 1u << 7  /* chords outside the LDS tile are staged per grid bin
+                                       (16384 voxels) in LDS, written as plain stores into
+                                       device pages and summed per bin after the launch (device
+                                       memory: ~8 B per chord, within SMCRT_TALLY_POOL_MB, default
+                                       half the free memory up to 64 GiB).  Neither flag: binned
+                                       where the scene's pilot run puts less than 35 % of the
+                                       path length into the tile.  Forward single-set launches
+                                       only; same result up to FP64 summation order             */
 };
 
 typedef struct smcrt_run_opts {
@@ -245,7 +254,8 @@ typedef struct smcrt_stats {
     uint64_t err_nan;        /* non-finite state caught and the packet dropped                */
     uint64_t ev_atomics;     /* FP64 tally atomics issued (chords merged per voxel)           */
     double moments[2][4][3]; /* SMCRT_FLAG_MOMENTS: sum of pos and pos^2 at scatters 1..4    */
-    double kernel_ms;        /* device time of the packet kernel                             */
+    double kernel_ms;        /* device time of the packet kernel (binned tally: through the
+                                bin sums)                                                     */
     double tally_sum;        /* sum of the FP64 fluence accumulator (raw path length)        */
 } smcrt_stats;
 

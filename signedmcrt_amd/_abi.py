@@ -30,6 +30,8 @@ FLAG_NO_TALLY = 1 << 2
 FLAG_NO_LDS_TILE = 1 << 3
 FLAG_WEIGHT = 1 << 4
 FLAG_SHARED_GRID = 1 << 5
+FLAG_DIRECT_TALLY = 1 << 6
+FLAG_BINNED_TALLY = 1 << 7
 
 MAX_REGIONS = 64
 MAX_NODES = 4096

@@ -8,7 +8,10 @@ Knobs (`SPACE`):
   walk_cap    voxel chords a lane walks per kernel-loop iteration before it sweeps again
   refill_min  regeneration policy: a wavefront refills its idle lanes once at least this many
               are idle (1 = at once)
-  tile        LDS-privatised FP32 tally tile on/off
+  tile        LDS-privatised FP64 tally tile on/off
+  tally       where chords outside the tile go: "direct" (FP64 atomics), "binned" (staged per grid
+              bin in LDS, written to device pages, summed per bin after the launch) or "auto"
+              (the library's choice from the scene's pilot run)
   waves       resident wavefronts per SIMD the kernel is built for (2, 3 or 4: fewer registers;
               4 only for the run-time compiled kernels, the library's run 3 instead)
 
@@ -40,11 +43,12 @@ SPACE: Dict[str, Tuple] = {
     "waves": (2, 3, 4),
     "block": (64, 128, 256),
     "tile": (True, False),
+    "tally": ("auto", "direct", "binned"),
     "chunk": (64, 128, 256, 512),
     "walk_cap": (1, 2, 4, 8, 16),
     "refill_min": (1, 8, 16, 32),
 }
-DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "waves": 3}
+DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "tally": "auto", "waves": 3}
 # counters that must not change with the launch shape
 INVARIANT = ("launched", "absorbed", "escaped_r", "escaped_t", "killed_surface", "killed_bounce",
              "scatters", "ev_steps", "ev_crossings", "ev_interfaces")
@@ -53,7 +57,8 @@ INVARIANT = ("launched", "absorbed", "escaped_r", "escaped_t", "killed_surface",
 def run_kwargs(shape: Dict) -> Dict:
     """Engine.run / run_device keywords for a shape dict."""
     kw = {k: int(shape[k]) for k in ("block", "chunk", "walk_cap", "refill_min", "waves") if k in shape}
-    kw["flags"] = 0 if shape.get("tile", True) else A.FLAG_NO_LDS_TILE
+    kw["flags"] = (0 if shape.get("tile", True) else A.FLAG_NO_LDS_TILE) | \
+        {"auto": 0, "direct": A.FLAG_DIRECT_TALLY, "binned": A.FLAG_BINNED_TALLY}[shape.get("tally", "auto")]
     return kw
 
 

@@ -122,8 +122,39 @@ struct KParams {
     // LDS-privatised tally tile: voxels [t0, t0 + tn) per axis accumulate in FP64 LDS per block
     // and are flushed into the FP64 grid at the end (tn[0] == 0: no tile)
     int t0[3], tn[3];
+    // Binned tally (kernels built with BIN, single-set forward launches): the grid's flat index
+    // range is cut into bins of 2^SMCRT_BIN_SHIFT voxels; entries of the nhot hot bins
+    // (binmap[bin] = slot, SMCRT_BIN_COLD otherwise) are staged per block in LDS, SMCRT_BIN_B to a
+    // slot, and a full slot is appended with plain stores to the slot's current page of `pool`
+    // (SMCRT_BIN_PAGE entries; block b takes pages from [b * ppb, (b + 1) * ppb)); the pages are
+    // summed per bin after the launch.  Other entries, and any once the block's pages run out,
+    // take the FP64 atomics.
+    int nbins, nhot;
+    const uint16_t* binmap;
+    uint2* pool;              // pages of (grid index, FP32 length) entries
+    uint32_t* page_bin;       // bin of each page
+    uint32_t* page_fill;      // entries in each page
+    uint32_t* block_pages;    // pages each block took
+    unsigned ppb;             // pages per block
     const struct KCold* cold;  // rarely used parameters (constant memory)
 };
+
+// Binned tally geometry (see KParams): bin size (voxels), entries per staged slot, page size
+#define SMCRT_BIN_SHIFT 14
+#define SMCRT_BIN_B 8
+#define SMCRT_BIN_PAGE 512
+#define SMCRT_BIN_COLD 0xFFFFu
+#define SMCRT_PAGE_NONE 0xFFFFFFFFu
+
+// LDS bytes of a block's binned-tally stage for `waves` wavefronts: the bin map (u16 per bin),
+// per hot bin a reserve and a write counter, its current page and that page's fill (u32 each),
+// the block's page counter (16 bytes), per wavefront 128 u32 of flush scratch, then the staged
+// entries
+__host__ __device__ inline size_t bins_map_bytes(int nbins) { return ((size_t)nbins * 2 + 15) & ~(size_t)15; }
+__host__ __device__ inline size_t bins_lds_bytes(int nbins, int nhot, int waves) {
+    return nhot > 0 ? bins_map_bytes(nbins) + 16 * (size_t)nhot + 16 + 512 * (size_t)waves + 8 * (size_t)SMCRT_BIN_B * nhot
+                    : 0;
+}
 
 struct Lane {
     float px, py, pz, dx, dy, dz;
@@ -608,15 +639,116 @@ SMCRT_DEV int classify_escape(CKCold* C, const Lane& L) {
 // lanes active drains them.  push/drain run in wave-converged code only (`n` is wave-uniform);
 // between calls the queue holds < 64 entries, so 128 entries per wavefront suffice.
 constexpr int TALLY_QUEUE = 128;
+
+// A block's binned-tally stage in LDS (KParams: nbins, nhot, ...), at `c`: per hot bin (slot)
+// cnt[slot] reserves places in the slot's buffer of SMCRT_BIN_B entries and wr[slot] counts the
+// entries written.  Every writer counts its entry after writing it (LDS executes a wavefront's
+// instructions in order), so the lane whose count completes the buffer knows it is full: it
+// flushes it and resets wr, then cnt; no lane waits on another wavefront.  Entries that find the
+// buffer full, or whose bin is cold, take an atomic.  pg[slot], fill[slot]: the slot's current
+// page and its entries (only the slot's flusher touches them, and flushes of one slot follow each
+// other through the reset).  Layout after the bin map:
+//   cnt[nhot], wr[nhot], pg[nhot], fill[nhot], used (the block's page counter) + 3 pad,
+//   scratch[128] per wavefront, buf[nhot][SMCRT_BIN_B]
+template <bool BIN>
 struct TallyQueue {
-    uint2* e;  // this wavefront's entries: (grid element index, FP32 length bits)
-    int n;     // entries held (wave-uniform)
+    uint2* e;     // this wavefront's entries: (grid element index, FP32 length bits)
+    int n;        // entries held (wave-uniform)
+    uint32_t* c;  // BIN: the stage's counters
 };
 
-SMCRT_DEV void tally_drain(TallyQueue& Q, double* __restrict__ base, int lane, bool all) {
+SMCRT_DEV void lds_order() { __asm__ volatile("" ::: "memory"); }
+
+// Hand lanes [0, n) of the queue to the binned stage (converged code: every lane present).
+// Returns the entries that will not take an atomic (staged, less those the block had no page for).
+SMCRT_DEV uint32_t bin_stage_drain(TallyQueue<true>& Q, const KParams& P, double* __restrict__ base, int lane, int n) {
+    const int nh = P.nhot;
+    uint32_t* const cnt = Q.c;
+    uint32_t* const wr = cnt + nh;
+    uint32_t* const pgs = cnt + 2 * nh;
+    uint32_t* const fills = cnt + 3 * nh;
+    uint32_t* const used = cnt + 4 * nh;
+    uint32_t* const scr = used + 4 + (threadIdx.x / 64) * 128;
+    uint2* const buf = reinterpret_cast<uint2*>(used + 4 + (blockDim.x / 64) * 128);
+    const uint16_t* const map = reinterpret_cast<const uint16_t*>(reinterpret_cast<unsigned char*>(cnt) - bins_map_bytes(P.nbins));
+    bool flusher = false, staged = false;
+    uint32_t info = 0u;  // slot | bin << 16 of a full buffer
+    if (lane < n) {
+        const uint2 e = Q.e[lane];
+        const uint32_t bin = e.x >> SMCRT_BIN_SHIFT;
+        const uint32_t slot = bin < (uint32_t)P.nbins ? (uint32_t)map[bin] : SMCRT_BIN_COLD;
+        bool direct = true;
+        if (slot != SMCRT_BIN_COLD) {
+            const uint32_t s = __hip_atomic_fetch_add(cnt + slot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (s < SMCRT_BIN_B) {
+                buf[slot * SMCRT_BIN_B + s] = e;
+                lds_order();
+                const uint32_t w = __hip_atomic_fetch_add(wr + slot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                flusher = w == SMCRT_BIN_B - 1;
+                info = slot | (bin << 16);
+                direct = false;
+                staged = true;
+            }
+        }
+        if (direct) unsafeAtomicAdd(base + e.x, (double)__uint_as_float(e.y));
+    }
+    uint32_t kept = (uint32_t)__builtin_popcountll(__ballot(staged));
+    const uint64_t fl = __ballot(flusher);
+    if (fl == 0ull) return kept;
+    // the full buffers, listed in the wavefront's scratch, eight at a time (lane = 8 * buffer +
+    // entry): the first lane of a buffer's group advances its slot's page (a new one from the
+    // block's range when it is full) and passes the place on through the scratch; the group
+    // writes the buffer there with one 64-byte store
+    const uint32_t nf = (uint32_t)__builtin_popcountll(fl);
+    if (flusher) scr[__builtin_amdgcn_mbcnt_hi((uint32_t)(fl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)fl, 0u))] = info;
+    lds_order();
+    const uint32_t pg0 = blockIdx.x * P.ppb;
+    for (uint32_t g0 = 0; g0 < nf; g0 += 64 / SMCRT_BIN_B) {
+        const uint32_t g = g0 + (uint32_t)lane / SMCRT_BIN_B, j = (uint32_t)lane % SMCRT_BIN_B;
+        const bool mine = g < nf;
+        const uint32_t inf = mine ? scr[g] : 0u;
+        const uint32_t kf = inf & 0xFFFFu;
+        if (mine && j == 0) {
+            uint32_t pg = pgs[kf], off = fills[kf];
+            if (pg == SMCRT_PAGE_NONE || off == SMCRT_BIN_PAGE) {
+                if (pg != SMCRT_PAGE_NONE) P.page_fill[pg] = SMCRT_BIN_PAGE;  // full
+                const uint32_t np = __hip_atomic_fetch_add(used, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                pg = np < P.ppb ? pg0 + np : SMCRT_PAGE_NONE;  // none left: the entries take atomics
+                off = 0u;
+                if (pg != SMCRT_PAGE_NONE) P.page_bin[pg] = inf >> 16;
+            }
+            pgs[kf] = pg;
+            fills[kf] = pg != SMCRT_PAGE_NONE ? off + SMCRT_BIN_B : 0u;
+            scr[64 + g % 64] = pg;
+        }
+        lds_order();
+        const uint32_t pg = mine ? scr[64 + g % 64] : SMCRT_PAGE_NONE;
+        const uint32_t off = mine && pg != SMCRT_PAGE_NONE ? fills[kf] - SMCRT_BIN_B : 0u;
+        const uint2 v = mine ? buf[kf * SMCRT_BIN_B + j] : make_uint2(0u, 0u);
+        if (mine) {
+            if (pg != SMCRT_PAGE_NONE) P.pool[(size_t)pg * SMCRT_BIN_PAGE + off + j] = v;
+            else unsafeAtomicAdd(base + v.x, (double)__uint_as_float(v.y));
+        }
+        kept -= (uint32_t)__builtin_popcountll(__ballot(mine && pg == SMCRT_PAGE_NONE));
+        lds_order();  // the buffer is read (in order before these writes): release it
+        if (mine && j == 0) {
+            wr[kf] = 0u;
+            lds_order();
+            cnt[kf] = 0u;
+        }
+    }
+    return kept;
+}
+
+// Returns the drained entries that did not take an atomic (binned stage).
+template <bool BIN>
+SMCRT_DEV uint32_t tally_drain(TallyQueue<BIN>& Q, const KParams& P, double* __restrict__ base, int lane, bool all) {
+    uint32_t kept = 0u;
     while (Q.n >= 64 || (all && Q.n > 0)) {
         const int n = Q.n < 64 ? Q.n : 64;
-        if (lane < n) {
+        if constexpr (BIN) {
+            kept += bin_stage_drain(Q, P, base, lane, n);
+        } else if (lane < n) {
             const uint2 e = Q.e[lane];
             unsafeAtomicAdd(base + e.x, (double)__uint_as_float(e.y));
         }
@@ -627,10 +759,14 @@ SMCRT_DEV void tally_drain(TallyQueue& Q, double* __restrict__ base, int lane, b
         }
         Q.n = rem;
     }
+    return kept;
 }
 
-// Returns the number of entries appended (wave-uniform).
-SMCRT_DEV unsigned tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, double* __restrict__ base, int lane) {
+// Returns the number of entries appended (wave-uniform), less those a drain staged (the count of
+// FP64 tally atomics, up to entries still queued).
+template <bool BIN>
+SMCRT_DEV unsigned tally_push(TallyQueue<BIN>& Q, const KParams& P, bool want, uint32_t idx, float v,
+                              double* __restrict__ base, int lane) {
     const uint64_t m = __ballot(want);
     if (m == 0ull) return 0u;
     if (want) {
@@ -639,7 +775,7 @@ SMCRT_DEV unsigned tally_push(TallyQueue& Q, bool want, uint32_t idx, float v, d
     }
     const unsigned cnt = (unsigned)__builtin_popcountll(m);
     Q.n += (int)cnt;
-    if (Q.n >= 64) tally_drain(Q, base, lane, false);
+    if (Q.n >= 64) return cnt - tally_drain(Q, P, base, lane, false);
     return cnt;
 }
 
@@ -681,8 +817,9 @@ struct InterpretedSweep {
 };
 
 // WAVES: resident waves per SIMD the registers are allocated for (2: at most 256 VGPRs, 3: at most
-// 168).  SW: the sweep policy.
-template <bool ADJ, uint32_t KM, int WAVES = 2, class SW = InterpretedSweep>
+// 168).  SW: the sweep policy.  BIN: the binned tally's stage (forward single-set launches of
+// the kernels compiled at run time; KParams).
+template <bool ADJ, uint32_t KM, int WAVES = 2, class SW = InterpretedSweep, bool BIN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) packet_kernel(KParams P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr bool kDiag = SW::kDiagnostics, kDet = SW::kDetectors, kWeight = SW::kWeight;
@@ -708,12 +845,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     // this block's tally target (its XCD's partial grid; workgroups go to XCDs round-robin)
     double* const tbase = P.xgrid ? P.xgrid + (size_t)(blockIdx.x % (unsigned)P.xcopies) * P.xstride : P.jmean;
     // each wavefront's tally queue comes last (after the tile and the adjoint's scores)
-    TallyQueue Q;
+    TallyQueue<BIN> Q;
     Q.n = 0;
+    Q.c = nullptr;
     {
         size_t qoff = lds_head + sizeof(double) * ((tile_vox + 1) & ~1);
         if (ADJ) qoff += sizeof(float) * 4 * P.nmat * blockDim.x + sizeof(double) * 4 * P.nmat;
         Q.e = reinterpret_cast<uint2*>(smem + qoff) + (threadIdx.x / 64) * TALLY_QUEUE;
+        // the binned-tally stage follows the queues: the bin map, then the counters (zeroed)
+        if constexpr (BIN) {
+            unsigned char* sb = smem + qoff + sizeof(uint2) * TALLY_QUEUE * (blockDim.x / 64);
+            uint16_t* map = reinterpret_cast<uint16_t*>(sb);
+            for (int i = threadIdx.x; i < P.nbins; i += blockDim.x) map[i] = P.binmap[i];
+            Q.c = reinterpret_cast<uint32_t*>(sb + bins_map_bytes(P.nbins));
+            const int nh = P.nhot;
+            for (int i = threadIdx.x; i < 4 * nh + 4; i += blockDim.x) Q.c[i] = (i >= 2 * nh && i < 3 * nh) ? SMCRT_PAGE_NONE : 0u;
+        }
     }
     AdjLds A;
     if (ADJ) {
@@ -1110,7 +1257,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         }
         if constexpr (kWaveOut) out[O_SCATTERS] += __builtin_popcountll(wave_ballot(scat));
         else out[O_SCATTERS] += scat ? 1u : 0u;
-        if (kWeight && P.jmean) ev[E_ATOMICS] += tally_push(Q, wpush, widx, wlen, tbase, lane_id);
+        if (kWeight && P.jmean) ev[E_ATOMICS] += tally_push(Q, P, wpush, widx, wlen, tbase, lane_id);
 
         // ---- march step (src/inttau2.f90:64-83): optical depth, then the voxel walk ----
         ev[E_STEPS] += __builtin_popcountll(wave_ballot(do_march));
@@ -1202,7 +1349,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 walking = steps;
             }
             ev[E_CROSS] += __builtin_popcountll(wave_ballot(crossed));
-            if (P.jmean) ev[E_ATOMICS] += tally_push(Q, gpush, gidx, glen, tbase, lane_id);
+            if (P.jmean) ev[E_ATOMICS] += tally_push(Q, P, gpush, gidx, glen, tbase, lane_id);
         }
 
         // ---- step finished: move, then end of flight (interaction next iteration) or next march step ----
@@ -1296,13 +1443,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             }
             L.st = ST_IDLE;
         }
-        if (P.jmean) ev[E_ATOMICS] += tally_push(Q, epush, eidx, elen, tbase, lane_id);
+        if (P.jmean) ev[E_ATOMICS] += tally_push(Q, P, epush, eidx, elen, tbase, lane_id);
     }
-    if (P.jmean) tally_drain(Q, tbase, lane_id, true);
+    if (P.jmean) ev[E_ATOMICS] -= tally_drain(Q, P, tbase, lane_id, true);
 
     // ---- flush the tally tile (single-set launches only) into the FP64 grid ----
+    if (tile_vox > 0 || BIN) __syncthreads();
     if (tile_vox > 0) {
-        __syncthreads();
         for (int i = threadIdx.x; i < tile_vox; i += blockDim.x) {
             const double v = s_tile[i];
             if (v != 0.0) {
@@ -1312,6 +1459,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                 unsafeAtomicAdd(tbase + g, v);
             }
         }
+    }
+    // ---- binned tally: entries left in the slots' buffers take atomics; the pages' fills ----
+    if constexpr (BIN) {
+        const int nh = P.nhot;
+        const uint32_t* cnt = Q.c;
+        const uint32_t* pgs = Q.c + 2 * nh;
+        const uint32_t* fills = Q.c + 3 * nh;
+        const uint32_t used = Q.c[4 * nh];
+        const uint2* buf = reinterpret_cast<const uint2*>(Q.c + 4 * nh + 4 + (blockDim.x / 64) * 128);
+        for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+            const uint32_t c = cnt[k] < SMCRT_BIN_B ? cnt[k] : SMCRT_BIN_B;
+            for (uint32_t i = 0; i < c; ++i) {
+                const uint2 v = buf[k * SMCRT_BIN_B + i];
+                unsafeAtomicAdd(tbase + v.x, (double)__uint_as_float(v.y));
+            }
+            if (pgs[k] != SMCRT_PAGE_NONE) P.page_fill[pgs[k]] = fills[k];
+        }
+        if (threadIdx.x == 0) P.block_pages[blockIdx.x] = used < P.ppb ? used : P.ppb;
     }
 
     // ---- flush counters ----

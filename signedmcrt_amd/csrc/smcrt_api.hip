@@ -72,6 +72,95 @@ __global__ void k_sum_f64(const double* __restrict__ p, size_t n, double* out) {
     if (threadIdx.x == 0) atomicAdd(out, part[0]);
 }
 
+__global__ void k_fill_u32(uint32_t* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// ---- binned tally: after the packet kernel, the pages of every bin are summed in LDS ----
+// Block b of the packet kernel took pages [b * ppb, b * ppb + block_pages[b]), each holding
+// page_fill entries of one bin.  One workgroup here per packet-kernel block counts its pages per
+// bin in LDS (at most 4096 bins) and adds the counts to the bins' totals:
+constexpr int BIN_MAX = 4096;
+__global__ void k_bin_count(const uint32_t* __restrict__ page_bin, const uint32_t* __restrict__ block_pages,
+                            unsigned ppb, int nbins, uint32_t* __restrict__ bin_pages) {
+    __shared__ uint32_t h[BIN_MAX];
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const size_t p0 = (size_t)blockIdx.x * ppb;
+    const uint32_t n = block_pages[blockIdx.x];
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) atomicAdd(h + page_bin[p0 + j], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+        if (h[i]) atomicAdd(bin_pages + i, h[i]);
+}
+// ... then lists the pages bin by bin (bin_off: each bin's first list position): the block's run
+// of each bin is reserved with one atomic, its pages placed with LDS atomics
+__global__ void k_bin_list(const uint32_t* __restrict__ page_bin, const uint32_t* __restrict__ block_pages,
+                           unsigned ppb, int nbins, const uint32_t* __restrict__ bin_off, uint32_t* __restrict__ bin_cur,
+                           uint32_t* __restrict__ list) {
+    __shared__ uint32_t h[BIN_MAX];  // counts, then the block's next list position per bin
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const size_t p0 = (size_t)blockIdx.x * ppb;
+    const uint32_t n = block_pages[blockIdx.x];
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) atomicAdd(h + page_bin[p0 + j], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+        if (h[i]) h[i] = bin_off[i] + atomicAdd(bin_cur + i, h[i]);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) list[atomicAdd(h + page_bin[p0 + j], 1u)] = (uint32_t)(p0 + j);
+}
+// One work item: list positions [l0, l1) of one bin's pages; `shared`: the bin has other items
+struct BinWork {
+    uint32_t bin, l0, l1, shared;
+};
+// Sum a work item's entries in an FP64 LDS tile of the bin's voxels, then add the tile to the
+// grid (plain read-modify-write when the item holds the whole bin, atomics otherwise).  The
+// item's page ids and fills are read into LDS first; each thread then keeps 8 entry loads in
+// flight (the loads are independent, so the pages stream at memory speed).
+constexpr int BIN_VOX = 1 << SMCRT_BIN_SHIFT;
+constexpr uint32_t BIN_PAGES_PER_ITEM = 2048;  // reduction work item: 1M entries
+__global__ void __launch_bounds__(1024) k_bin_accum(const BinWork* __restrict__ work, const uint32_t* __restrict__ list,
+                                                    const uint2* __restrict__ pool, const uint32_t* __restrict__ page_fill,
+                                                    double* __restrict__ grid, size_t nvox) {
+    __shared__ double t[BIN_VOX];
+    __shared__ uint32_t pid[BIN_PAGES_PER_ITEM], pfill[BIN_PAGES_PER_ITEM];
+    const BinWork w = work[blockIdx.x];
+    const uint32_t np = w.l1 - w.l0;  // <= BIN_PAGES_PER_ITEM
+    for (int i = threadIdx.x; i < BIN_VOX; i += blockDim.x) t[i] = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+        const uint32_t p = list[w.l0 + i];
+        pid[i] = p;
+        pfill[i] = page_fill[p];
+    }
+    __syncthreads();
+    const uint32_t n = np * SMCRT_BIN_PAGE;
+    constexpr int U = 8;
+    for (uint32_t j0 = threadIdx.x; j0 < n; j0 += U * blockDim.x) {
+        uint2 e[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = j0 + (uint32_t)u * blockDim.x;
+            const uint32_t k = j < n ? j / SMCRT_BIN_PAGE : 0u, i = j % SMCRT_BIN_PAGE;
+            ok[u] = j < n && i < pfill[k];
+            e[u] = ok[u] ? pool[(size_t)pid[k] * SMCRT_BIN_PAGE + i] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) atomicAdd(t + (e[u].x & (BIN_VOX - 1)), (double)__uint_as_float(e[u].y));
+    }
+    __syncthreads();
+    const size_t g0 = (size_t)w.bin << SMCRT_BIN_SHIFT;
+    for (int i = threadIdx.x; i < BIN_VOX; i += blockDim.x) {
+        const double v = t[i];
+        if (v != 0.0 && g0 + i < nvox) {
+            if (w.shared) unsafeAtomicAdd(grid + g0 + i, v);
+            else grid[g0 + i] += v;
+        }
+    }
+}
+
 // dst[i] += sum over the partial grids c of src[c * n + i] (per-XCD tallies, fixed order)
 __global__ void k_reduce_copies(double* __restrict__ dst, const double* __restrict__ src, size_t n, int copies) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -161,7 +250,8 @@ struct DevBuf {
 struct JitKernels {
     bool tried = false, ok = false, cached = false;
     hipModule_t mod = nullptr;
-    hipFunction_t fwd[3] = {nullptr, nullptr, nullptr}, adj[3] = {nullptr, nullptr, nullptr};  // [waves - 2]
+    // [waves - 2]: forward, adjoint, forward with the binned tally's stage
+    hipFunction_t fwd[3] = {nullptr, nullptr, nullptr}, adj[3] = {nullptr, nullptr, nullptr}, bin[3] = {nullptr, nullptr, nullptr};
     std::string error;
     double seconds = 0.0;
 };
@@ -191,6 +281,19 @@ struct smcrt_scene {
     // pilot run on first use
     bool tile_ready = false;
     int tile_t0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, tile_tn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    // binned tally plan, from the same pilot run, per occupancy of 2, 3 and 4 blocks per CU: the
+    // hot bins' slots (SMCRT_BIN_COLD for the others), their number, the tile that fits beside
+    // their LDS stage, the share of the pilot's path length in the hot bins and in the direct
+    // tally's tile; tally entries per packet (merged chords, no tile) in the pilot
+    int nbins = 0;
+    std::vector<uint16_t> binmap[3];
+    int nhot[3] = {0, 0, 0};
+    int btile_t0[3][3] = {}, btile_tn[3][3] = {};
+    double hot_frac[3] = {0.0, 0.0, 0.0}, tile_capture[3] = {0.0, 0.0, 0.0};
+    double pilot_entries = 0.0;
+    int binmap_up = -1;  // which occupancy's map d_binmap holds
+    std::vector<uint32_t> h_bin_pages, h_bin_off;
+    std::vector<BinWork> h_bin_work;
     // device
     int device = -1;
     int num_cu = 0;
@@ -201,7 +304,8 @@ struct smcrt_scene {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf d_nodes, d_regions, d_mats, d_region_mat, d_mat_mus, d_work, d_stats, d_moments,
         d_grad, d_setinfo, d_jmean, d_weights, d_trace, d_pts, d_out, d_sum, d_cold, d_det, d_det_counts, d_accel,
-        d_adj_mats, d_adj_mus, d_xgrid;
+        d_adj_mats, d_adj_mus, d_xgrid, d_binmap, d_pool, d_page_bin, d_page_fill, d_block_pages, d_bin_pages,
+        d_bin_off, d_bin_cur, d_bin_list, d_bin_work;
     // host copies of the region-indexed material table and per-material mu_s the allocation-free
     // entry point uploads from (kept alive across its asynchronous copies; rewritten only when
     // the materials change)
@@ -685,7 +789,9 @@ extern "C" int smcrt_scene_destroy(smcrt_scene* s) {
         DevBuf* bufs[] = {&s->d_nodes, &s->d_regions, &s->d_mats, &s->d_region_mat, &s->d_mat_mus, &s->d_work,
                           &s->d_stats, &s->d_moments, &s->d_grad, &s->d_setinfo, &s->d_jmean, &s->d_weights, &s->d_xgrid,
                           &s->d_trace, &s->d_pts, &s->d_out, &s->d_sum, &s->d_cold, &s->d_det,
-                          &s->d_det_counts, &s->d_accel, &s->d_adj_mats, &s->d_adj_mus};
+                          &s->d_det_counts, &s->d_accel, &s->d_adj_mats, &s->d_adj_mus, &s->d_binmap, &s->d_pool,
+                          &s->d_page_bin, &s->d_page_fill, &s->d_block_pages, &s->d_bin_pages, &s->d_bin_off,
+                          &s->d_bin_cur, &s->d_bin_list, &s->d_bin_work};
         for (DevBuf* b : bufs) b->release();
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -829,6 +935,7 @@ static void fill_common(const smcrt_scene* s, KParams& P, KCold& Cc) {
 struct LaunchShape {
     int block, blocks, chunk, walk_cap, refill_min, waves;
     bool compiled;  // the scene's run-time compiled kernels (JitKernels) instead of the library's
+    bool bin;       // the compiled forward kernel with the binned tally's stage
     size_t lds;
 };
 
@@ -917,9 +1024,10 @@ static void jit_array(std::string& o, const std::string& name, const float* v, i
 }
 
 // Instantiation name expressions: [adj][waves - 2]
-static std::string jit_name(uint32_t km, bool adj, int waves) {
-    return std::string("smcrt::packet_kernel<") + (adj ? "true" : "false") + ", " + std::to_string(km) + "u, " +
-           std::to_string(waves) + ", smcrt::SceneSweep>";
+// kind 0: forward, 1: adjoint, 2: forward with the binned tally's stage
+static std::string jit_name(uint32_t km, int kind, int waves) {
+    return std::string("smcrt::packet_kernel<") + (kind == 1 ? "true" : "false") + ", " + std::to_string(km) + "u, " +
+           std::to_string(waves) + ", smcrt::SceneSweep, " + (kind == 2 ? "true" : "false") + ">";
 }
 
 static std::string jit_source(const smcrt_scene* s, uint32_t km) {
@@ -1009,8 +1117,8 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
         o += "    }\n";
     }
     o += "};\n";
-    for (int adj = 0; adj < 2; ++adj)
-        for (int w = 2; w <= 4; ++w) o += "template __global__ void " + jit_name(km, adj, w).substr(7) + "(KParams);\n";
+    for (int kind = 0; kind < 3; ++kind)
+        for (int w = 2; w <= 4; ++w) o += "template __global__ void " + jit_name(km, kind, w).substr(7) + "(KParams);\n";
     o += "}  // namespace smcrt\n";
     return o;
 }
@@ -1038,7 +1146,7 @@ static std::string jit_cache_dir() {
 
 // cache file: u32 count, then per name (u32 length, bytes), then u64 code size and the code
 // kernels in a compiled module: forward and adjoint, each built for 2, 3 and 4 waves
-static constexpr uint32_t kJitKernels = 6;
+static constexpr uint32_t kJitKernels = 9;
 
 static bool jit_cache_read(const std::string& path, std::vector<std::string>& names, std::vector<char>& code) {
     FILE* f = fopen(path.c_str(), "rb");
@@ -1149,9 +1257,9 @@ static bool jit_build(smcrt_scene* s) {
             return false;
         }
         std::vector<std::string> exprs;
-        for (int adj = 0; adj < 2; ++adj)
+        for (int kind = 0; kind < 3; ++kind)
             for (int w = 2; w <= 4; ++w) {
-                exprs.push_back(jit_name(km, adj, w));
+                exprs.push_back(jit_name(km, kind, w));
                 (void)hiprtcAddNameExpression(prog, exprs.back().c_str());
             }
         const hiprtcResult rc = hiprtcCompileProgram(prog, nopts, opts);
@@ -1189,7 +1297,8 @@ static bool jit_build(smcrt_scene* s) {
         J.mod = nullptr;
         return false;
     }
-    hipFunction_t* fn[kJitKernels] = {&J.fwd[0], &J.fwd[1], &J.fwd[2], &J.adj[0], &J.adj[1], &J.adj[2]};
+    hipFunction_t* fn[kJitKernels] = {&J.fwd[0], &J.fwd[1], &J.fwd[2], &J.adj[0], &J.adj[1], &J.adj[2],
+                                      &J.bin[0], &J.bin[1], &J.bin[2]};
     if (names.size() != kJitKernels) {
         J.error = "unexpected kernel count in the compiled module";
         (void)hipModuleUnload(J.mod);
@@ -1222,21 +1331,22 @@ static bool jit_wanted(const smcrt_run_opts* o) {
     return o && o->n_packets >= (1ull << 22);
 }
 
-static hipFunction_t jit_function(const smcrt_scene* s, bool adj, int waves) {
-    return adj ? s->jit.adj[waves - 2] : s->jit.fwd[waves - 2];
+static hipFunction_t jit_function(const smcrt_scene* s, bool adj, int waves, bool bin = false) {
+    return adj ? s->jit.adj[waves - 2] : (bin ? s->jit.bin[waves - 2] : s->jit.fwd[waves - 2]);
 }
 
 static hipError_t kernel_occupancy(const smcrt_scene* s, bool adj, int waves, bool compiled, int block, size_t lds,
-                                   int* active) {
-    if (compiled) return hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(active, jit_function(s, adj, waves), block, lds);
+                                   int* active, bool bin = false) {
+    if (compiled)
+        return hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(active, jit_function(s, adj, waves, bin), block, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(active, packet_kernel_for(s, adj, waves), block, lds);
 }
 
 // 256-lane blocks per CU the kernel's registers allow, clamped to 2..4: the index + 2 of the tally
 // tile sized for that occupancy (the 2-wave build of a small scene program can fit 4 waves).
-static int register_blocks(const smcrt_scene* s, bool adj, int waves, bool compiled) {
+static int register_blocks(const smcrt_scene* s, bool adj, int waves, bool compiled, bool bin = false) {
     int active = 0;
-    if (kernel_occupancy(s, adj, waves, compiled, 256, 0, &active) != hipSuccess || active < 2) return 2;
+    if (kernel_occupancy(s, adj, waves, compiled, 256, 0, &active, bin) != hipSuccess || active < 2) return 2;
     return active > 4 ? 4 : active;
 }
 
@@ -1244,7 +1354,7 @@ static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const Lau
                                        KParams& P) {
     void* args[] = {&P};
     if (sh.compiled)
-        return hipModuleLaunchKernel(jit_function(s, adj, sh.waves), (unsigned)sh.blocks, 1, 1, (unsigned)sh.block, 1, 1,
+        return hipModuleLaunchKernel(jit_function(s, adj, sh.waves, sh.bin), (unsigned)sh.blocks, 1, 1, (unsigned)sh.block, 1, 1,
                                      (unsigned)sh.lds, stream, args, nullptr);
     return hipLaunchKernel((const void*)packet_kernel_for(s, adj, sh.waves), dim3(sh.blocks), dim3(sh.block), args,
                            sh.lds, stream);
@@ -1253,9 +1363,10 @@ static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const Lau
 // Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
 // material tables of n_sets sets, plus the adjoint's per-lane scores and gradients).
 static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0,
-                              int adj_slots = 0, bool compiled = false) {
+                              int adj_slots = 0, bool compiled = false, int nhot = 0) {
     LaunchShape L;
     L.compiled = compiled;
+    L.bin = compiled && !adj && nhot > 0;
     L.block = (o && o->block_size > 0) ? o->block_size : 256;
     int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
@@ -1270,7 +1381,8 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (L.block > 256) L.block = 256;
     // per lane: the wavefront's tally queue (TALLY_QUEUE 8-byte entries per 64 lanes)
     const size_t queue_lane = sizeof(uint2) * TALLY_QUEUE / 64;
-    L.lds = mats + queue_lane * L.block;
+    // the binned tally's stage follows the queues
+    L.lds = mats + queue_lane * L.block + (L.bin ? bins_lds_bytes(s->nbins, nhot, L.block / 64) : 0);
     if (adj) {
         // per lane: 4 FP32 scores per adjoint material; per block: 4 FP64 sums per material
         const size_t per_lane = (size_t)16 * (size_t)adj_slots + queue_lane, per_block = (size_t)32 * (size_t)adj_slots;
@@ -1280,7 +1392,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once
         int active = 0;
-        hipError_t e = kernel_occupancy(s, adj, L.waves, compiled, L.block, L.lds, &active);
+        hipError_t e = kernel_occupancy(s, adj, L.waves, compiled, L.block, L.lds, &active, L.bin);
         if (e == hipSuccess && active > 0) bpc = active;
     }
     L.blocks = s->num_cu * bpc;
@@ -1333,10 +1445,11 @@ struct AdjTables {
 static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, bool adj, const float* d_w,
                       hipStream_t stream, float* kernel_ms, const AdjTables* at = nullptr);
 
-// Pick the LDS tally tile from a pilot run of the scene (once per scene).
+// Pick the LDS tally tile and the binned tally's hot bins from a pilot run of the scene (once per
+// scene).
 static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stream) {
     if (s->tile_ready) return SMCRT_OK;
-    s->tile_ready = true;  // the pilot itself runs without a tile
+    s->tile_ready = true;  // the pilot itself runs without a tile or bins
     const size_t nv = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
     double* pilot = nullptr;
     if (hipMalloc(&pilot, nv * sizeof(double)) != hipSuccess) return SMCRT_OK;  // no tile, not an error
@@ -1344,31 +1457,164 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
     smcrt_run_opts po = *o;
     po.n_packets = 100000;
     po.first_packet = (1ull << 62);
-    po.flags = 0;
+    po.flags = SMCRT_FLAG_DIRECT_TALLY;
     po.trace_packets = 0;
     po.trace_events = 0;
     float ms = 0.f;
     int rc = launch_run(s, &po, pilot, false, nullptr, stream, &ms);
     std::vector<double> h(nv);
-    if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
-        // one tile per register occupancy of 2, 3 or 4 256-lane blocks per CU (a launch takes the
-        // one its kernel's registers allow): the LDS left per block, at most 64 KiB of FP64 sums
+    unsigned long long ev[NOUT + NEV];
+    if (!rc && hipMemcpy(h.data(), pilot, nv * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+        hipMemcpy(ev, s->d_stats.p, sizeof ev, hipMemcpyDeviceToHost) == hipSuccess) {
+        double total = 0.0;
+        for (size_t i = 0; i < nv; ++i) total += h[i];
+        auto box_mass = [&](const int t0[3], const int tn[3]) {
+            double m = 0.0;
+            for (int k = t0[2]; k < t0[2] + tn[2]; ++k)
+                for (int j = t0[1]; j < t0[1] + tn[1]; ++j)
+                    for (int i = t0[0]; i < t0[0] + tn[0]; ++i)
+                        m += h[(size_t)i + (size_t)s->grid.nx * ((size_t)j + (size_t)s->grid.ny * k)];
+            return m;
+        };
+        // bins of 2^SMCRT_BIN_SHIFT voxels by the pilot's path length (heaviest first)
+        const size_t nb = (nv + BIN_VOX - 1) >> SMCRT_BIN_SHIFT;
+        std::vector<double> bmass;
+        std::vector<int> order;
+        if (nb <= 4096 && nv < (1ull << 32) && total > 0.0) {
+            bmass.assign(nb, 0.0);
+            for (size_t i = 0; i < nv; ++i) bmass[i >> SMCRT_BIN_SHIFT] += h[i];
+            for (size_t b = 0; b < nb; ++b)
+                if (bmass[b] > 0.0) order.push_back((int)b);
+            std::sort(order.begin(), order.end(), [&](int a, int b) { return bmass[a] > bmass[b]; });
+            s->nbins = (int)nb;
+            s->pilot_entries = (double)ev[NOUT + E_ATOMICS] / (double)po.n_packets;
+        }
+        // per register occupancy of 2, 3 or 4 256-lane blocks per CU (a launch takes the one its
+        // kernel's registers allow): the LDS left per block, a tile of at most 64 KiB of FP64 sums;
+        // for the binned tally, hot bins in at most 60 % of that LDS and a tile in the rest
         int lds_cu = 0;
         if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
             lds_cu <= 0)
             lds_cu = 160 * 1024;
         for (int bpc = 2; bpc <= 4; ++bpc) {
+            const int wi = bpc - 2;
             const long mats = (long)s->regions.size() * (long)sizeof(DevMat) +
                               (long)sizeof(GradDesc) * grad_desc_count((int)s->regions.size());
             const long queue = (long)(sizeof(uint2) * TALLY_QUEUE) * (256 / 64);
             const long per_block = (long)lds_cu / bpc - mats - queue - 256;
             const int budget = (int)std::max(0L, std::min(8192L, per_block / (long)sizeof(double)));
-            if (budget > 0)
-                choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0[bpc - 2], s->tile_tn[bpc - 2]);
+            if (budget > 0) {
+                choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, budget, s->tile_t0[wi], s->tile_tn[wi]);
+                s->tile_capture[wi] = total > 0.0 ? box_mass(s->tile_t0[wi], s->tile_tn[wi]) / total : 0.0;
+            }
+            if (s->nbins > 0 && per_block > 0) {
+                const long slot = 16 + 8 * SMCRT_BIN_B;
+                const long kmax = ((long)(0.6 * per_block) - (long)bins_lds_bytes(s->nbins, 1, 4) + slot) / slot;
+                const int k = (int)std::max(0L, std::min({kmax, (long)order.size(), (long)SMCRT_BIN_COLD - 1}));
+                s->binmap[wi].assign(nb, (uint16_t)SMCRT_BIN_COLD);
+                double hm = 0.0;
+                for (int j = 0; j < k; ++j) {
+                    s->binmap[wi][order[j]] = (uint16_t)j;
+                    hm += bmass[order[j]];
+                }
+                s->nhot[wi] = k;
+                s->hot_frac[wi] = hm / total;
+                const long left = per_block - (long)bins_lds_bytes(s->nbins, k, 4);
+                const int bbudget = (int)std::max(0L, std::min(8192L, left / (long)sizeof(double)));
+                if (k > 0 && bbudget > 0)
+                    choose_tile(h.data(), s->grid.nx, s->grid.ny, s->grid.nz, bbudget, s->btile_t0[wi], s->btile_tn[wi]);
+            }
         }
     }
     (void)hipFree(pilot);
     return rc;
+}
+
+// Binned tally: pages for a launch (the pilot's tally entries per packet in the hot bins x 1.25,
+// plus one partly filled page per hot bin and block), within SMCRT_TALLY_POOL_MB or half the free
+// device memory (at most 64 GiB); the bin map of the launch's occupancy; the kernel's pointers.
+// Entries beyond a block's pages take the FP64 atomics, so a short pool only costs speed.
+static int bin_pool(smcrt_scene* s, const smcrt_run_opts* o, const LaunchShape& sh, int wi, KParams& P,
+                    hipStream_t stream) {
+    const int nh = s->nhot[wi], nb = s->nbins;
+    const double want = s->pilot_entries * s->hot_frac[wi] * (double)o->n_packets * 1.25;
+    size_t pages = (size_t)(want / SMCRT_BIN_PAGE) + (size_t)sh.blocks * (size_t)(nh + 1);
+    const size_t per_page = (size_t)SMCRT_BIN_PAGE * sizeof(uint2) + 3 * sizeof(uint32_t);  // pool, bin, fill, list
+    size_t cap = 0;
+    if (const char* e = getenv("SMCRT_TALLY_POOL_MB"); e && *e) {
+        cap = (size_t)strtoull(e, nullptr, 10) << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+        const size_t held = s->d_pool.bytes + s->d_page_bin.bytes + s->d_page_fill.bytes + s->d_bin_list.bytes;
+        cap = std::min((size_t)64 << 30, (fr + held) / 2);
+    }
+    pages = std::min({pages, cap / per_page, (size_t)0xFFFFFFFFu});
+    const size_t ppb = pages / (size_t)sh.blocks;
+    pages = ppb * (size_t)sh.blocks;
+    int rc = 0;
+    if ((rc = s->d_pool.ensure(std::max<size_t>(pages, 1) * SMCRT_BIN_PAGE * sizeof(uint2))) ||
+        (rc = s->d_page_bin.ensure(std::max<size_t>(pages, 1) * sizeof(uint32_t))) ||
+        (rc = s->d_page_fill.ensure(std::max<size_t>(pages, 1) * sizeof(uint32_t))) ||
+        (rc = s->d_bin_list.ensure(std::max<size_t>(pages, 1) * sizeof(uint32_t))) ||
+        (rc = s->d_block_pages.ensure((size_t)sh.blocks * sizeof(uint32_t))) ||
+        (rc = s->d_bin_pages.ensure((size_t)nb * sizeof(uint32_t))) ||
+        (rc = s->d_bin_off.ensure((size_t)nb * sizeof(uint32_t))) ||
+        (rc = s->d_bin_cur.ensure((size_t)nb * sizeof(uint32_t))) ||
+        (rc = s->d_bin_work.ensure(((size_t)nb + pages / BIN_PAGES_PER_ITEM + 1) * sizeof(BinWork))) ||
+        (rc = s->d_binmap.ensure((size_t)nb * sizeof(uint16_t))))
+        return rc;
+    if (s->binmap_up != wi) {
+        HIPCHECK(hipMemcpyAsync(s->d_binmap.p, s->binmap[wi].data(), (size_t)nb * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                stream));
+        s->binmap_up = wi;
+    }
+    P.nbins = nb;
+    P.nhot = nh;
+    P.binmap = s->d_binmap.as<uint16_t>();
+    P.pool = s->d_pool.as<uint2>();
+    P.page_bin = s->d_page_bin.as<uint32_t>();
+    P.page_fill = s->d_page_fill.as<uint32_t>();
+    P.block_pages = s->d_block_pages.as<uint32_t>();
+    P.ppb = (unsigned)ppb;
+    return SMCRT_OK;
+}
+
+// Binned tally after the launch: pages per bin (read back), the pages listed bin by bin, then one
+// LDS-tile workgroup per item of up to 1M entries of a bin adds the sums into the grid.
+static int bin_reduce(smcrt_scene* s, const LaunchShape& sh, const KParams& P, double* d_jmean, size_t nvox,
+                      hipStream_t stream) {
+    const int nb = s->nbins;
+    uint32_t* const bp = s->d_bin_pages.as<uint32_t>();
+    k_fill_u32<<<(unsigned)fill_blocks((size_t)nb), 256, 0, stream>>>(bp, (size_t)nb);
+    k_bin_count<<<(unsigned)sh.blocks, 256, 0, stream>>>(P.page_bin, P.block_pages, P.ppb, nb, bp);
+    HIPCHECK(hipGetLastError());
+    s->h_bin_pages.assign(nb, 0u);
+    HIPCHECK(hipMemcpyAsync(s->h_bin_pages.data(), bp, (size_t)nb * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    s->h_bin_off.assign(nb, 0u);
+    s->h_bin_work.clear();
+    uint32_t tot = 0;
+    for (int b = 0; b < nb; ++b) {
+        const uint32_t c = s->h_bin_pages[b];
+        s->h_bin_off[b] = tot;
+        for (uint32_t j = 0; j < c; j += BIN_PAGES_PER_ITEM)
+            s->h_bin_work.push_back(
+                BinWork{(uint32_t)b, tot + j, tot + std::min(j + BIN_PAGES_PER_ITEM, c), c > BIN_PAGES_PER_ITEM ? 1u : 0u});
+        tot += c;
+    }
+    if (s->h_bin_work.empty()) return SMCRT_OK;
+    if (s->h_bin_work.size() * sizeof(BinWork) > s->d_bin_work.bytes) return fail(SMCRT_EDEVICE, "bin work list overflow");
+    HIPCHECK(hipMemcpyAsync(s->d_bin_off.p, s->h_bin_off.data(), (size_t)nb * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(s->d_bin_work.p, s->h_bin_work.data(), s->h_bin_work.size() * sizeof(BinWork),
+                            hipMemcpyHostToDevice, stream));
+    k_fill_u32<<<(unsigned)fill_blocks((size_t)nb), 256, 0, stream>>>(s->d_bin_cur.as<uint32_t>(), (size_t)nb);
+    k_bin_list<<<(unsigned)sh.blocks, 256, 0, stream>>>(P.page_bin, P.block_pages, P.ppb, nb, s->d_bin_off.as<uint32_t>(),
+                                                        s->d_bin_cur.as<uint32_t>(), s->d_bin_list.as<uint32_t>());
+    k_bin_accum<<<(unsigned)s->h_bin_work.size(), 1024, 0, stream>>>(s->d_bin_work.as<BinWork>(), s->d_bin_list.as<uint32_t>(),
+                                                                     P.pool, P.page_fill, d_jmean, nvox);
+    HIPCHECK(hipGetLastError());
+    return SMCRT_OK;
 }
 
 // Launch the packet kernel for one scene (one set).  d_jmean may be null (no tally).
@@ -1380,7 +1626,8 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     }
     if (o->trace_packets < 0 || o->trace_events < 0) return fail(SMCRT_EINVAL, "negative trace sizes");
     const bool use_tile = !adj && d_jmean && !(o->flags & SMCRT_FLAG_NO_LDS_TILE);
-    if (use_tile) {
+    const bool may_bin = !adj && d_jmean && !(o->flags & SMCRT_FLAG_DIRECT_TALLY);
+    if (use_tile || may_bin) {
         int rc = ensure_tile(s, o, stream);
         if (rc) return rc;
     }
@@ -1432,12 +1679,29 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     // the scene's run-time compiled kernels (built once per scene; on failure the library's)
     const bool compiled = jit_wanted(o) && jit_build(s);
     int tile_vox = 0;
+    bool binned = false;
+    int wi = 0;
+    const int waves = compiled ? compiled_waves(o) : launch_waves(s, o, adj);
+    if (use_tile || may_bin) wi = register_blocks(s, adj, waves, compiled) - 2;
+    // binned tally (the compiled kernels' stage): asked for, or by default where the tile holds
+    // little of the path length; the plan of the binned kernel's own occupancy
+    if (may_bin && compiled && s->nbins > 0 && s->nhot[wi] > 0 &&
+        ((o->flags & SMCRT_FLAG_BINNED_TALLY) || s->tile_capture[wi] < 0.35)) {
+        wi = register_blocks(s, adj, waves, compiled, true) - 2;
+        binned = s->nhot[wi] > 0;
+    }
     if (use_tile) {
-        const int wi = register_blocks(s, adj, compiled ? compiled_waves(o) : launch_waves(s, o, adj), compiled) - 2;
-        for (int a = 0; a < 3; ++a) { P.t0[a] = s->tile_t0[wi][a]; P.tn[a] = s->tile_tn[wi][a]; }
+        for (int a = 0; a < 3; ++a) {
+            P.t0[a] = binned ? s->btile_t0[wi][a] : s->tile_t0[wi][a];
+            P.tn[a] = binned ? s->btile_tn[wi][a] : s->tile_tn[wi][a];
+        }
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }
-    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat, compiled);
+    LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat, compiled, binned ? s->nhot[wi] : 0);
+    if (binned) {
+        int rc = bin_pool(s, o, sh, wi, P, stream);
+        if (rc) return rc;
+    }
     P.chunk = sh.chunk;
     P.dda_cap = sh.walk_cap;
     P.refill_min = sh.refill_min;
@@ -1464,8 +1728,13 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     HIPCHECK(hipEventRecord(s->ev0, stream));
     HIPCHECK(launch_packet_kernel(s, adj, sh, stream, P));
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(s->ev1, stream));
+    if (!binned) HIPCHECK(hipEventRecord(s->ev1, stream));
     if (copies) k_reduce_copies<<<(unsigned)fill_blocks(nvox), 256, 0, stream>>>(d_jmean, P.xgrid, nvox, copies);
+    if (binned) {
+        int rc = bin_reduce(s, sh, P, d_jmean, nvox, stream);
+        if (rc) return rc;
+        HIPCHECK(hipEventRecord(s->ev1, stream));
+    }
     HIPCHECK(hipStreamSynchronize(stream));
     HIPCHECK(hipEventElapsedTime(kernel_ms, s->ev0, s->ev1));
     return SMCRT_OK;

@@ -105,8 +105,10 @@ def kernel_label(eng, waves, n_packets):
     name, scene_mask, variant_mask = eng.kernel_variant()
     jit = os.environ.get("SMCRT_JIT", "")
     if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= (1 << 22))):
-        return "packet_kernel<false, %du, %d, SceneSweep> (%s)" % (scene_mask, waves, name)
-    return "packet_kernel<false, %du, %d, InterpretedSweep> (%s)" % (variant_mask, min(waves, 3), name)
+        # the last BIN parameter: the binned tally's stage (the library reports the last launch's)
+        return "packet_kernel<false, %du, %d, SceneSweep, %s> (%s)" % (
+            scene_mask, waves, "true" if "+binned" in name else "false", name)
+    return "packet_kernel<false, %du, %d, InterpretedSweep, false> (%s)" % (variant_mask, min(waves, 3), name)
 
 
 def read_traffic(workload, label):
@@ -430,7 +432,7 @@ def bench_batched(args):
     achieved = flops / (replay_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-            "kernel": "packet_kernel<false, %du, %d, InterpretedSweep> x %d geometry groups (one graph replay)" % (
+            "kernel": "packet_kernel<false, %du, %d, InterpretedSweep, false> x %d geometry groups (one graph replay)" % (
                 engines[0].kernel_variant()[2], min(int(shape.get("waves", 3)), 3), len(engines)),
             "kernel_ms": replay_ms, "flops_per_launch": flops}
     roof_tally = {"bound": "hbm", "achieved": tally / (replay_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,

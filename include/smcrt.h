@@ -392,7 +392,9 @@ int64_t smcrt_detector_bins(const smcrt_scene* scene, int32_t i);
    is compiled for a few sets of primitive/operator/modifier kinds; a scene runs the smallest
    one covering the kinds its program uses.  Writes the scene's kind mask (bit k: primitive k;
    bit 15+k: operator k; bit 23+k: modifier k) and the variant's mask, and returns its name
-   ("basic", "layers", "extruded" or "full"; NULL for a NULL scene). */
+   ("basic", "layers", "extruded" or "full"; NULL for a NULL scene), followed by "+compiled" or
+   "+compiled(cached)" once the scene's kernels were compiled at run time, and "+binned" when the
+   scene's last forward launch ran the binned tally (SMCRT_FLAG_BINNED_TALLY). */
 const char* smcrt_scene_kernel(const smcrt_scene* scene, uint32_t* scene_mask, uint32_t* variant_mask);
 
 /* ---- input: the reference's TOML schema (src/parse/parse.f90, parse_source.f90,

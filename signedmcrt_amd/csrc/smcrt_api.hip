@@ -292,6 +292,7 @@ struct smcrt_scene {
     double hot_frac[3] = {0.0, 0.0, 0.0}, tile_capture[3] = {0.0, 0.0, 0.0};
     double pilot_entries = 0.0;
     int binmap_up = -1;  // which occupancy's map d_binmap holds
+    bool last_binned = false;  // the last forward launch ran the binned tally
     std::vector<uint32_t> h_bin_pages, h_bin_off;
     std::vector<BinWork> h_bin_work;
     // device
@@ -1491,7 +1492,8 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
         }
         // per register occupancy of 2, 3 or 4 256-lane blocks per CU (a launch takes the one its
         // kernel's registers allow): the LDS left per block, a tile of at most 64 KiB of FP64 sums;
-        // for the binned tally, hot bins in at most 60 % of that LDS and a tile in the rest
+        // for the binned tally, hot bins in at most 60 % of that LDS (90 % where the direct
+        // tally's tile holds under 10 % of the path length) and a tile in the rest
         int lds_cu = 0;
         if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) != hipSuccess ||
             lds_cu <= 0)
@@ -1508,8 +1510,10 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
                 s->tile_capture[wi] = total > 0.0 ? box_mass(s->tile_t0[wi], s->tile_tn[wi]) / total : 0.0;
             }
             if (s->nbins > 0 && per_block > 0) {
+                // where the tile holds little (< 10 % of the path length) the bins take 90 %
+                const double share = s->tile_capture[wi] < 0.1 ? 0.9 : 0.6;
                 const long slot = 16 + 8 * SMCRT_BIN_B;
-                const long kmax = ((long)(0.6 * per_block) - (long)bins_lds_bytes(s->nbins, 1, 4) + slot) / slot;
+                const long kmax = ((long)(share * per_block) - (long)bins_lds_bytes(s->nbins, 1, 4) + slot) / slot;
                 const int k = (int)std::max(0L, std::min({kmax, (long)order.size(), (long)SMCRT_BIN_COLD - 1}));
                 s->binmap[wi].assign(nb, (uint16_t)SMCRT_BIN_COLD);
                 double hm = 0.0;
@@ -1698,6 +1702,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
         tile_vox = P.tn[0] * P.tn[1] * P.tn[2];
     }
     LaunchShape sh = pick_shape(s, o, adj, 1, tile_vox, P.nmat, compiled, binned ? s->nhot[wi] : 0);
+    if (!adj && d_jmean) s->last_binned = binned;
     if (binned) {
         int rc = bin_pool(s, o, sh, wi, P, stream);
         if (rc) return rc;
@@ -2457,5 +2462,6 @@ extern "C" const char* smcrt_scene_kernel(const smcrt_scene* s, uint32_t* scene_
     m->kernel_name = v.name;
     if (s->jit.ok) m->kernel_name += s->jit.cached ? "+compiled(cached)" : "+compiled";
     else if (s->jit.tried) m->kernel_name += "+compile-failed: " + s->jit.error;
+    if (s->last_binned) m->kernel_name += "+binned";
     return m->kernel_name.c_str();
 }

@@ -188,7 +188,8 @@ def main():
         tuned = {"shape": eng.shape, "given": True}
     elif not args.no_tune:
         t_tune = time.perf_counter()
-        res = eng.tune(use_cache=False, adjoint=bool(grad_spec and args.grad_steps > 0))
+        res = eng.tune(use_cache=False, adjoint=bool(grad_spec and args.grad_steps > 0),
+                       adjoint_materials=grad_spec[0] if grad_spec else None)
         tuned = {"shape": res.shape, "probe_kernel_ms": res.kernel_ms,
                  "default_probe_kernel_ms": res.default_ms, "speedup_vs_default": res.speedup,
                  "points": len(res.table), "seconds": time.perf_counter() - t_tune}

@@ -204,8 +204,11 @@ enum {
                                        memory: ~8 B per chord, within SMCRT_TALLY_POOL_MB, default
                                        half the free memory up to 64 GiB).  Neither flag: binned
                                        where the scene's pilot run puts less than 35 % of the
-                                       path length into the tile.  Forward single-set launches
-                                       only; same result up to FP64 summation order             */
+                                       path length into the tile and hands the global tally more
+                                       than 1.5 entries per sweep.  Forward single-set launches
+                                       of the run-time compiled kernels only (the library's
+                                       kernels take the atomics); same result up to FP64
+                                       summation order                                          */
 };
 
 typedef struct smcrt_run_opts {

@@ -142,7 +142,8 @@ def _device_name() -> str:
 def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
          cache_path: Optional[str] = None, use_cache: bool = True,
          space: Optional[Dict[str, Sequence]] = None, seed: int = 7919,
-         target_ms: float = 150.0, min_gain: float = 0.02, adjoint: bool = True) -> TuneResult:
+         target_ms: float = 150.0, min_gain: float = 0.02, adjoint: bool = True,
+         adjoint_materials: Optional[Sequence[int]] = None) -> TuneResult:
     """Tune the launch shape of `engine`'s scene on the current GPU.
 
     n_probe: packets per probe (default: enough for `target_ms` of kernel time at the default
@@ -152,7 +153,9 @@ def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
     dominate.  A knob moves away from the library default only for a gain above `min_gain`.
     The probe tallies into a scratch device grid.  With `adjoint`, the adjoint kernel's resident
     waves are tuned as well, on the forward's best shape (its register budget differs: it keeps
-    per-material scores), and returned as the shape's "adj_waves" (used by Engine.grad)."""
+    per-material scores), and returned as the shape's "adj_waves" (used by Engine.grad), with
+    the materials `adjoint_materials` differentiated (default every material, or the first 16:
+    an adjoint launch tracks at most 16)."""
     import torch
 
     space = dict(space or SPACE)
@@ -204,12 +207,15 @@ def tune(engine, n_probe: Optional[int] = None, reps: int = 2, sweeps: int = 2,
         # same packets at every point: the counters must agree across the adjoint's probes
         adj: Dict[int, float] = {}
         adj_ref: Dict = {}
+        mats = adjoint_materials
+        if mats is None and engine.scene.n_materials > 16:
+            mats = list(range(16))
         for w in space["waves"]:
             kw = run_kwargs(dict(best, waves=w))
             kw.pop("flags")  # the LDS tile option: the adjoint does not tally
             t = float("inf")
             for _ in range(reps):
-                st = engine.grad(n_probe, seed=seed, first_packet=1 << 50, **kw).stats
+                st = engine.grad(n_probe, seed=seed, first_packet=1 << 50, materials=mats, **kw).stats
                 inv = {k: st[k] for k in INVARIANT}
                 if adj_ref and adj_ref != inv:
                     raise RuntimeError(f"adjoint at {w} waves changed the transported work: {inv} != {adj_ref}")

@@ -290,7 +290,7 @@ struct smcrt_scene {
     int nhot[3] = {0, 0, 0};
     int btile_t0[3][3] = {}, btile_tn[3][3] = {};
     double hot_frac[3] = {0.0, 0.0, 0.0}, tile_capture[3] = {0.0, 0.0, 0.0};
-    double pilot_entries = 0.0;
+    double pilot_entries = 0.0, pilot_entries_per_sweep = 0.0;
     int binmap_up = -1;  // which occupancy's map d_binmap holds
     bool last_binned = false;  // the last forward launch ran the binned tally
     std::vector<uint32_t> h_bin_pages, h_bin_off;
@@ -1489,6 +1489,7 @@ static int ensure_tile(smcrt_scene* s, const smcrt_run_opts* o, hipStream_t stre
             std::sort(order.begin(), order.end(), [&](int a, int b) { return bmass[a] > bmass[b]; });
             s->nbins = (int)nb;
             s->pilot_entries = (double)ev[NOUT + E_ATOMICS] / (double)po.n_packets;
+            s->pilot_entries_per_sweep = (double)ev[NOUT + E_ATOMICS] / std::max(1.0, (double)ev[NOUT + E_SWEEPS]);
         }
         // per register occupancy of 2, 3 or 4 256-lane blocks per CU (a launch takes the one its
         // kernel's registers allow): the LDS left per block, a tile of at most 64 KiB of FP64 sums;
@@ -1688,9 +1689,11 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     const int waves = compiled ? compiled_waves(o) : launch_waves(s, o, adj);
     if (use_tile || may_bin) wi = register_blocks(s, adj, waves, compiled) - 2;
     // binned tally (the compiled kernels' stage): asked for, or by default where the tile holds
-    // little of the path length; the plan of the binned kernel's own occupancy
+    // little of the path length and the tally is heavy next to the sweeps (more than 1.5 global
+    // entries per sweep in the pilot: the atomics then bound the launch); the plan of the binned
+    // kernel's own occupancy
     if (may_bin && compiled && s->nbins > 0 && s->nhot[wi] > 0 &&
-        ((o->flags & SMCRT_FLAG_BINNED_TALLY) || s->tile_capture[wi] < 0.35)) {
+        ((o->flags & SMCRT_FLAG_BINNED_TALLY) || (s->tile_capture[wi] < 0.35 && s->pilot_entries_per_sweep > 1.5))) {
         wi = register_blocks(s, adj, waves, compiled, true) - 2;
         binned = s->nhot[wi] > 0;
     }

@@ -168,6 +168,13 @@ template <class T>
 struct V3 {
     T x, y, z;
 };
+
+// plain float (the sweep's evaluations) as opposed to Dual numbers (gradients)
+template <class T> struct is_plain_float { static constexpr bool value = false; };
+template <> struct is_plain_float<float> { static constexpr bool value = true; };
+// keeps a rarely taken wave-uniform branch a branch (the compiler would otherwise compute its
+// body for every wavefront and select)
+SMCRT_DEV void rare_path() { __asm__ volatile(""); }
 template <class T> SMCRT_DEV T vdot(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <class T> SMCRT_DEV V3<T> vsub(const V3<T>& a, const V3<T>& b) { return V3<T>{a.x - b.x, a.y - b.y, a.z - b.z}; }
 template <class T> SMCRT_DEV V3<T> vsubc(const V3<T>& a, float bx, float by, float bz) { return V3<T>{a.x - bx, a.y - by, a.z - bz}; }
@@ -263,6 +270,19 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const float* __rest
     }
     case 2: if constexpr (!KM_HAS(KM, KM_PRIM(2))) break; {  // box
         V3<T> q = vsubc(vabs(p), P[0], P[1], P[2]);
+        if constexpr (is_plain_float<T>::value) {
+            // length(max(q, 0)) + min(max(q), 0) is max(q) unless two or more components are
+            // positive (near an edge or corner): the square root only where a lane needs it
+            float d = fmaxf(q.x, fmaxf(q.y, q.z));
+            const bool edge = __builtin_amdgcn_fmed3f(q.x, q.y, q.z) > 0.f;  // two or more positive
+            if (__builtin_amdgcn_ballot_w64(edge)) {
+                rare_path();
+                const float mx = fmaxf(q.x, 0.f), my = fmaxf(q.y, 0.f), mz = fmaxf(q.z, 0.f);
+                const float e = f_sqrt(mx * mx + my * my + mz * mz);
+                d = edge ? e : d;
+            }
+            return d;
+        }
         return vlen(vmax0(q)) + g_min0(g_max(q.x, g_max(q.y, q.z)));
     }
     case 3: if constexpr (!KM_HAS(KM, KM_PRIM(3))) break; {  // torus
@@ -283,6 +303,18 @@ SMCRT_DEV T prim_eval(int kind, const float* __restrict__ P, const float* __rest
             const float L = ax == 1 ? bax : (ax == 2 ? bay : baz);  // signed length along the axis
             const T X = g_sqrt(u * u + v * v) - P[6];
             const T Y = g_abs(w - 0.5f * L) - 0.5f * fabsf(L);
+            if constexpr (is_plain_float<T>::value) {
+                // max(X, Y) unless both are positive (beyond the rim of a cap): the square root
+                // only where a lane needs it
+                float d = fmaxf(X, Y);
+                const bool rim = (X > 0.f) & (Y > 0.f);
+                if (__builtin_amdgcn_ballot_w64(rim)) {
+                    rare_path();
+                    const float e = f_sqrt(X * X + Y * Y);
+                    d = rim ? e : d;
+                }
+                return d;
+            }
             const float xv = val(X), yv = val(Y);
             if (fmaxf(xv, yv) < 0.f) return g_max(X, Y);
             const T mx = g_max0(X), my = g_max0(Y);

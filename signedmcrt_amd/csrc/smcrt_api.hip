@@ -360,7 +360,8 @@ static bool point_in_polygon(const std::vector<double>& X, const std::vector<dou
     return in;
 }
 
-// Build the accelerator of one polygon (n vertices at xy) and append it to A; returns its base.
+// Build the accelerator of one polygon (n vertices at xy) and append it to A; returns its base
+// (a multiple of 4 floats, so every record the device reads as a 16-byte vector is aligned).
 // Grid cells are about one mean edge length (8..512 cells across the polygon).
 static int build_polygon_accel(const double* xy, int n, std::vector<float>& A) {
     std::vector<double> X(n), Y(n);
@@ -387,16 +388,18 @@ static int build_polygon_accel(const double* xy, int n, std::vector<float>& A) {
     const int nx = (int)std::ceil((xmax + m - gx0) / cell), ny = (int)std::ceil((ymax + m - gy0) / cell);
     const double r = cell * std::sqrt(0.5);
     const double slack = 1e-5 * std::fmax(W, 1.0);
+    A.resize((A.size() + 3) & ~(size_t)3, 0.f);
     const int base = (int)A.size();
-    const int eoff = base + 16, coff = eoff + 12 * n, loff = coff + 4 * nx * ny;
+    const int eoff = base + 16, coff = eoff + 8 * n, loff = coff + 4 * nx * ny;
     A.resize(loff, 0.f);
+    auto put_int = [](float* f, int v) { memcpy(f, &v, sizeof v); };
     float* H = A.data() + base;
     H[0] = (float)nx; H[1] = (float)ny; H[2] = (float)gx0; H[3] = (float)gy0;
     H[4] = (float)(1.0 / cell); H[5] = (float)cell;
     H[6] = (float)(gx0 + nx * cell); H[7] = (float)(gy0 + ny * cell);
     H[8] = (float)m; H[9] = (float)eoff; H[10] = (float)coff; H[11] = (float)loff; H[12] = (float)n;
     for (int i = 0; i < n; ++i) {
-        float* E = A.data() + eoff + 12 * i;
+        float* E = A.data() + eoff + 8 * i;
         const int ip = (i + n - 1) % n, in = (i + 1) % n;
         double n0x = NX[ip] + NX[i], n0y = NY[ip] + NY[i];  // vertex pseudo-normals (bisectors)
         double n1x = NX[i] + NX[in], n1y = NY[i] + NY[in];
@@ -404,11 +407,11 @@ static int build_polygon_accel(const double* xy, int n, std::vector<float>& A) {
         if (l0 < 1e-12) { n0x = NX[i]; n0y = NY[i]; l0 = 1; }
         if (l1 < 1e-12) { n1x = NX[i]; n1y = NY[i]; l1 = 1; }
         E[0] = (float)X[i]; E[1] = (float)Y[i]; E[2] = (float)EX[i]; E[3] = (float)EY[i];
-        E[4] = (float)(1.0 / (EX[i] * EX[i] + EY[i] * EY[i]));
-        E[5] = (float)(n0x / l0); E[6] = (float)(n0y / l0); E[7] = (float)(n1x / l1); E[8] = (float)(n1y / l1);
+        E[4] = (float)(n0x / l0); E[5] = (float)(n0y / l0); E[6] = (float)(n1x / l1); E[7] = (float)(n1y / l1);
     }
     std::vector<double> dist(n);
     std::vector<float> lists;
+    std::vector<int> cand;
     for (int iy = 0; iy < ny; ++iy)
         for (int ix = 0; ix < nx; ++ix) {
             const double cx = gx0 + (ix + 0.5) * cell, cy = gy0 + (iy + 0.5) * cell;
@@ -419,34 +422,44 @@ static int build_polygon_accel(const double* xy, int n, std::vector<float>& A) {
             }
             const bool in = point_in_polygon(X, Y, cx, cy);
             float* Cc = A.data() + coff + 4 * (ix + nx * iy);
+            Cc[2] = (float)(in ? -D : D);
             if (D > 2.0 * r + slack) {  // far: no edge within the cell
-                Cc[0] = 0.f;
-                Cc[1] = 0.f;
-                Cc[2] = (float)(in ? -D : D);
-            } else {
-                Cc[0] = (float)(loff + lists.size());
-                int cnt = 0;
-                for (int i = 0; i < n; ++i)
-                    if (dist[i] <= D + 2.0 * r + slack) {
-                        lists.push_back((float)(eoff + 12 * i));
-                        ++cnt;
-                    }
-                Cc[1] = (float)cnt;
-                Cc[2] = (float)(in ? -D : D);
+                put_int(Cc + 0, 0);
+                put_int(Cc + 1, 0);
+                continue;
+            }
+            cand.clear();
+            for (int i = 0; i < n; ++i)
+                if (dist[i] <= D + 2.0 * r + slack) cand.push_back(eoff + 8 * i);
+            // pairs of edge offsets, an odd list padded with its first edge (a repeated edge never
+            // replaces the nearest: the device keeps the first of equal distances); the first pair
+            // sits in the cell record, the rest in the lists
+            if (cand.size() % 2) cand.push_back(cand[0]);
+            const int np = (int)cand.size() / 2;
+            put_int(Cc + 0, np > 1 ? loff + (int)lists.size() : 0);
+            put_int(Cc + 1, np);
+            put_int(Cc + 2, cand[0]);
+            put_int(Cc + 3, cand[1]);
+            for (size_t k = 2; k < cand.size(); ++k) {
+                float f;
+                memcpy(&f, &cand[k], sizeof f);
+                lists.push_back(f);
             }
         }
     A.insert(A.end(), lists.begin(), lists.end());
-    // every index the device follows must stay inside this polygon's records
+    // every offset the device follows must stay inside this polygon's records
     const size_t end = A.size();
+    auto get_int = [&](size_t i) { int v; memcpy(&v, A.data() + i, sizeof v); return v; };
+    auto edge_ok = [&](int e) { return e >= eoff && e + 8 <= coff && (e - eoff) % 8 == 0; };
     for (int c = 0; c < nx * ny; ++c) {
-        const float* Cc = A.data() + coff + 4 * c;
-        const size_t st = (size_t)Cc[0], cnt = (size_t)Cc[1];
-        if (cnt == 0) continue;
-        if (st < (size_t)loff || st + cnt > end) return -1;
-        for (size_t k = 0; k < cnt; ++k) {
-            const size_t e = (size_t)A[st + k];
-            if (e < (size_t)eoff || e + 12 > (size_t)coff || (e - eoff) % 12 != 0) return -1;
-        }
+        const size_t ci = (size_t)coff + 4 * (size_t)c;
+        const int rest = get_int(ci), np = get_int(ci + 1);
+        if (np == 0) continue;
+        if (np < 0 || !edge_ok(get_int(ci + 2)) || !edge_ok(get_int(ci + 3))) return -1;
+        if (np == 1) continue;
+        if (rest < loff || (rest - loff) % 2 != 0 || (size_t)rest + 2 * (size_t)(np - 1) > end) return -1;
+        for (int k = 0; k < 2 * (np - 1); ++k)
+            if (!edge_ok(get_int((size_t)rest + k))) return -1;
     }
     return base;
 }
@@ -661,7 +674,7 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
                         return fail(SMCRT_EINVAL, "region %d node %d: polygon vertex %d non-finite or repeated", r, i, v);
                     }
                 }
-                if (s->accel.size() + 16 + 12 * c > (1u << 24)) {
+                if (s->accel.size() + 20 + 8 * c > (1u << 24)) {
                     delete s;
                     return fail(SMCRT_ERANGE, "polygon accelerators exceed 2^24 floats");
                 }

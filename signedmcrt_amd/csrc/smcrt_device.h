@@ -84,21 +84,21 @@ SMCRT_DEV Dual operator*(const Dual& a, const Dual& b) {
     return Dual{a.v * b.v, a.x * b.v + a.v * b.x, a.y * b.v + a.v * b.y, a.z * b.v + a.v * b.z};
 }
 SMCRT_DEV Dual operator/(const Dual& a, const Dual& b) {
-    float r = 1.0f / b.v;
+    float r = f_rcp(b.v);
     float q = a.v * r;
     return Dual{q, (a.x - q * b.x) * r, (a.y - q * b.y) * r, (a.z - q * b.z) * r};
 }
-SMCRT_DEV Dual operator/(const Dual& a, float b) { return a * (1.0f / b); }
+SMCRT_DEV Dual operator/(const Dual& a, float b) { return a * f_rcp(b); }
 SMCRT_DEV Dual d_sqrt(const Dual& a) {
-    float s = sqrtf(a.v);
-    float h = s > 0.f ? 0.5f / s : 0.f;
+    float s = f_sqrt(a.v);
+    float h = s > 0.f ? 0.5f * f_rcp(s) : 0.f;
     return Dual{s, a.x * h, a.y * h, a.z * h};
 }
 SMCRT_DEV Dual d_abs(const Dual& a) { return a.v < 0.f ? -a : a; }
 SMCRT_DEV Dual d_min(const Dual& a, const Dual& b) { return b.v < a.v ? b : a; }
 SMCRT_DEV Dual d_max(const Dual& a, const Dual& b) { return b.v > a.v ? b : a; }
-SMCRT_DEV Dual d_sin(const Dual& a) { float s = sinf(a.v), c = cosf(a.v); return Dual{s, c * a.x, c * a.y, c * a.z}; }
-SMCRT_DEV Dual d_cos(const Dual& a) { float s = sinf(a.v), c = cosf(a.v); return Dual{c, -s * a.x, -s * a.y, -s * a.z}; }
+SMCRT_DEV Dual d_sin(const Dual& a) { float s = f_sin(a.v), c = f_cos(a.v); return Dual{s, c * a.x, c * a.y, c * a.z}; }
+SMCRT_DEV Dual d_cos(const Dual& a) { float s = f_sin(a.v), c = f_cos(a.v); return Dual{c, -s * a.x, -s * a.y, -s * a.z}; }
 
 // D1: value and one directional derivative.  Three passes (seed x, y, z) give the gradient at a
 // third of Dual's register footprint; the packet kernel uses it for interface normals.
@@ -116,15 +116,15 @@ SMCRT_DEV D1 operator*(const D1& a, float s) { return D1{a.v * s, a.d * s}; }
 SMCRT_DEV D1 operator*(float s, const D1& a) { return D1{a.v * s, a.d * s}; }
 SMCRT_DEV D1 operator*(const D1& a, const D1& b) { return D1{a.v * b.v, a.d * b.v + a.v * b.d}; }
 SMCRT_DEV D1 operator/(const D1& a, const D1& b) {
-    float r = 1.0f / b.v, q = a.v * r;
+    float r = f_rcp(b.v), q = a.v * r;
     return D1{q, (a.d - q * b.d) * r};
 }
-SMCRT_DEV D1 operator/(const D1& a, float b) { return a * (1.0f / b); }
+SMCRT_DEV D1 operator/(const D1& a, float b) { return a * f_rcp(b); }
 
 // generic wrappers so one template body serves float, Dual and D1
 SMCRT_DEV D1 g_sqrt(const D1& a) {
-    float s = sqrtf(a.v);
-    return D1{s, s > 0.f ? a.d * (0.5f / s) : 0.f};
+    float s = f_sqrt(a.v);
+    return D1{s, s > 0.f ? a.d * (0.5f * f_rcp(s)) : 0.f};
 }
 SMCRT_DEV D1 g_abs(const D1& a) { return a.v < 0.f ? -a : a; }
 SMCRT_DEV D1 g_min(const D1& a, const D1& b) { return b.v < a.v ? b : a; }
@@ -133,8 +133,8 @@ SMCRT_DEV D1 g_max0(const D1& a) { return a.v > 0.f ? a : D1{0.f, 0.f}; }
 SMCRT_DEV D1 g_min0(const D1& a) { return a.v < 0.f ? a : D1{0.f, 0.f}; }
 SMCRT_DEV D1 g_clamp01(const D1& a) { return a.v < 0.f ? D1{0.f, 0.f} : (a.v > 1.f ? D1{1.f, 0.f} : a); }
 SMCRT_DEV D1 g_div(const D1& a, const D1& b) { return a / b; }
-SMCRT_DEV D1 g_sin(const D1& a) { return D1{sinf(a.v), cosf(a.v) * a.d}; }
-SMCRT_DEV D1 g_cos(const D1& a) { return D1{cosf(a.v), -sinf(a.v) * a.d}; }
+SMCRT_DEV D1 g_sin(const D1& a) { return D1{f_sin(a.v), f_cos(a.v) * a.d}; }
+SMCRT_DEV D1 g_cos(const D1& a) { return D1{f_cos(a.v), -f_sin(a.v) * a.d}; }
 SMCRT_DEV D1 g_neg_if(const D1& a, bool neg) { return neg ? -a : a; }
 SMCRT_DEV float g_sqrt(float a) { return f_sqrt(a); }
 SMCRT_DEV Dual g_sqrt(const Dual& a) { return d_sqrt(a); }
@@ -184,19 +184,27 @@ template <class T> SMCRT_DEV V3<T> vmax0(const V3<T>& a) { return V3<T>{g_max0(a
 
 // ------------------------------------------------------------------------------------------
 // Closed polygon in the xy plane through its uniform-grid accelerator (built on the host by
-// smcrt_scene_create).  Accelerator layout, floats from `base`:
+// smcrt_scene_create).  Accelerator layout, floats from `base` (a multiple of 4: every record
+// below is read as aligned vectors):
 //   header[16]: nx, ny, gx0, gy0, 1/cell, cell, gx1, gy1, margin, edges, cells, lists, n_edges
-//   edges[12 each]: vx, vy, ex, ey, 1/|e|^2, start-vertex pseudo-normal (2), end-vertex
-//                   pseudo-normal (2), pad
-//   cells[4 each]: list start, list length (0: far cell), signed centre distance, pad
-//   lists: absolute float index of each candidate edge record
+//   edges[8 each]: vx, vy, ex, ey | start-vertex pseudo-normal (2), end-vertex pseudo-normal (2)
+//   cells[4 each]: far cell:  0, 0, signed centre distance, 0
+//                  near cell: rest (int bits), pairs (int bits, >= 1), first pair of candidate
+//                  edge offsets (int bits)
+//   lists: the near cells' further pairs of candidate edge offsets (absolute float offsets of
+//          edge records, int bits); an odd list is padded with its first edge
 // A near cell lists every edge that can be nearest anywhere in the cell (distance from the
 // centre <= D_c + 2r, r the half diagonal), so its distance is exact; its sign comes from the
 // nearest feature (edge side, or the vertex pseudo-normal when the nearest point is a vertex).
 // A far cell (D_c > 2r, no edge inside) returns sign * (D_c - |q - c|); outside the grid the
 // distance to the grid rectangle plus the margin — both lower bounds, which keep sphere
-// tracing safe and are only used away from the surface.
+// tracing safe and are only used away from the surface.  Latency, not arithmetic, bounds this
+// evaluation (a few L1/L2 hits per polygon per sweep): the cell record carries the first pair
+// of offsets, a pair's two 16-byte edge loads are in flight together, and the nearest edge's
+// pseudo-normals are read once, after the loop.
 // ------------------------------------------------------------------------------------------
+SMCRT_DEV float4 ld_f4(const float* __restrict__ A, int i) { return *reinterpret_cast<const float4*>(A + i); }
+
 template <class T>
 SMCRT_DEV T polygon_eval(const float* __restrict__ A, int base, const V3<T>& p) {
     const float* H = A + base;
@@ -211,34 +219,48 @@ SMCRT_DEV T polygon_eval(const float* __restrict__ A, int base, const V3<T>& p) 
     int ix = (int)((qx - gx0) * inv), iy = (int)((qy - gy0) * inv);
     ix = ix < nx - 1 ? ix : nx - 1;
     iy = iy < ny - 1 ? iy : ny - 1;
-    const float* Cc = A + (int)H[10] + 4 * (ix + nx * iy);
-    const int start = (int)Cc[0], cnt = (int)Cc[1];
-    if (cnt == 0) {
-        const float sD = Cc[2];
+    const float4 c = ld_f4(A, (int)H[10] + 4 * (ix + nx * iy));
+    const int np = __float_as_int(c.y);
+    if (np == 0) {
+        const float sD = c.z;
         T ddx = p.x - (gx0 + ((float)ix + 0.5f) * cell), ddy = p.y - (gy0 + ((float)iy + 0.5f) * cell);
         T r = fabsf(sD) - g_sqrt(ddx * ddx + ddy * ddy);
         return g_neg_if(r, sD < 0.f);
     }
+    const int rest = __float_as_int(c.x);
+    int o0 = __float_as_int(c.z), o1 = __float_as_int(c.w);
     T best = g_const<T>(3.0e38f);
     float bestv = 3.0e38f;
-    bool inside = false;
-    for (int k = 0; k < cnt; ++k) {
-        const float* E = A + (int)A[start + k];
-        const float vx = E[0], vy = E[1], ex = E[2], ey = E[3];
-        T wx = p.x - vx, wy = p.y - vy;
-        T t = g_clamp01((wx * ex + wy * ey) * E[4]);
-        T bx = wx - t * ex, by = wy - t * ey;
+    int be = o0;  // the nearest edge's record (a valid record even if no distance compares)
+    auto edge = [&](const float4& e, int off) {
+        T wx = p.x - e.x, wy = p.y - e.y;
+        T t = g_clamp01((wx * e.z + wy * e.w) * f_rcp(e.z * e.z + e.w * e.w));
+        T bx = wx - t * e.z, by = wy - t * e.w;
         T d2 = bx * bx + by * by;
         const float d2v = val(d2);
         if (d2v < bestv) {
             bestv = d2v;
             best = d2;
-            const float tv = val(t), wxv = val(wx), wyv = val(wy);
-            if (tv <= 0.f) inside = wxv * E[5] + wyv * E[6] < 0.f;
-            else if (tv >= 1.f) inside = (wxv - ex) * E[7] + (wyv - ey) * E[8] < 0.f;
-            else inside = ex * wyv - ey * wxv > 0.f;  // left of a counter-clockwise edge
+            be = off;
         }
+    };
+    for (int k = 0;;) {
+        const float4 e0 = ld_f4(A, o0), e1 = ld_f4(A, o1);
+        edge(e0, o0);
+        edge(e1, o1);
+        if (++k >= np) break;
+        const int2 v = *reinterpret_cast<const int2*>(A + rest + 2 * (k - 1));
+        o0 = v.x;
+        o1 = v.y;
     }
+    // the sign from the nearest feature of the nearest edge
+    const float4 e = ld_f4(A, be), pn = ld_f4(A, be + 4);
+    const float wxv = qx - e.x, wyv = qy - e.y;
+    const float tv = (wxv * e.z + wyv * e.w) * f_rcp(e.z * e.z + e.w * e.w);
+    bool inside;
+    if (tv <= 0.f) inside = wxv * pn.x + wyv * pn.y < 0.f;
+    else if (tv >= 1.f) inside = (wxv - e.z) * pn.z + (wyv - e.w) * pn.w < 0.f;
+    else inside = e.z * wyv - e.w * wxv > 0.f;  // left of a counter-clockwise edge
     return g_neg_if(g_sqrt(best), inside);
 }
 

@@ -111,6 +111,17 @@ def kernel_label(eng, waves, n_packets):
     return "packet_kernel<false, %du, %d, InterpretedSweep, false> (%s)" % (variant_mask, min(waves, 3), name)
 
 
+def batch_kernel_label(eng, waves, n_packets):
+    """The kernel a batch's geometry group of n_packets ran: the scene's run-time compiled kernel
+    for several sets (groups of >= 2^22 packets, once built; SMCRT_JIT=1: every group, =0: none)
+    or the library's variant."""
+    name, scene_mask, variant_mask = eng.kernel_variant()
+    jit = os.environ.get("SMCRT_JIT", "")
+    if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= (1 << 22))):
+        return "packet_kernel<false, %du, %d, SceneSweepSets, false>" % (scene_mask, waves)
+    return "packet_kernel<false, %du, %d, InterpretedSweep, false>" % (variant_mask, min(waves, 3))
+
+
 def read_traffic(workload, label):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary of this workload, when the
     summary's kernel is the one this run timed (its `kernel:` line equals `label` up to the
@@ -433,8 +444,9 @@ def bench_batched(args):
     achieved = flops / (replay_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-            "kernel": "packet_kernel<false, %du, %d, InterpretedSweep, false> x %d geometry groups (one graph replay)" % (
-                engines[0].kernel_variant()[2], min(int(shape.get("waves", 3)), 3), len(engines)),
+            "kernel": " + ".join(batch_kernel_label(eng, int(shape.get("waves", 3)),
+                                                    sum(s.n_packets for s in sets if s.scene == geo))
+                                 for geo, eng in enumerate(engines)) + " (one graph replay)",
             "kernel_ms": replay_ms, "flops_per_launch": flops}
     roof_tally = {"bound": "hbm", "achieved": tally / (replay_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": tally / (replay_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

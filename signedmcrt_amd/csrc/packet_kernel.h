@@ -809,6 +809,7 @@ struct InterpretedSweep {
     static constexpr bool kDetectors = true;    // detector records and the ghost continuation
     static constexpr bool kWeight = true;       // the weight_scatter estimator (SMCRT_FLAG_WEIGHT)
     static constexpr int kSource = -1;          // source kind (-1: read at run time)
+    static constexpr bool kSets = true;         // launches may hold several sets (batches)
     // a compiled policy overwrites the scene's fixed launch parameters (grid, epsilons, region
     // count) with constants
     static SMCRT_DEV void bake(KParams&) {}
@@ -890,9 +891,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     int pend_idx = -1;  // merged chord waiting for its atomic (grid index)
     int pend_li = -1;   // its tile index, -1 outside the tile
     float pend_len = 0.f;
-    // Outcome counters: per lane in the library's kernels (a lane's set can differ from its
-    // neighbours' in batched launches), wave-uniform ballot counts in single-set compiled ones.
-    constexpr bool kWaveOut = SW::kCompiled;
+    // Outcome counters: per lane where a lane's set can differ from its neighbours' (batched
+    // launches), wave-uniform ballot counts in single-set compiled kernels.
+    constexpr bool kWaveOut = SW::kCompiled && !SW::kSets;
     unsigned out[NOUT];
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) out[i] = 0;

@@ -250,8 +250,9 @@ struct DevBuf {
 struct JitKernels {
     bool tried = false, ok = false, cached = false;
     hipModule_t mod = nullptr;
-    // [waves - 2]: forward, adjoint, forward with the binned tally's stage
-    hipFunction_t fwd[3] = {nullptr, nullptr, nullptr}, adj[3] = {nullptr, nullptr, nullptr}, bin[3] = {nullptr, nullptr, nullptr};
+    // [waves - 2]: forward, adjoint, forward with the binned tally's stage, forward over several sets
+    hipFunction_t fwd[3] = {nullptr, nullptr, nullptr}, adj[3] = {nullptr, nullptr, nullptr}, bin[3] = {nullptr, nullptr, nullptr},
+                  sets[3] = {nullptr, nullptr, nullptr};
     std::string error;
     double seconds = 0.0;
 };
@@ -950,6 +951,7 @@ struct LaunchShape {
     int block, blocks, chunk, walk_cap, refill_min, waves;
     bool compiled;  // the scene's run-time compiled kernels (JitKernels) instead of the library's
     bool bin;       // the compiled forward kernel with the binned tally's stage
+    bool sets;      // the compiled forward kernel over several sets (batches)
     size_t lds;
 };
 
@@ -1037,11 +1039,13 @@ static void jit_array(std::string& o, const std::string& name, const float* v, i
     o += "};\n";
 }
 
-// Instantiation name expressions: [adj][waves - 2]
-// kind 0: forward, 1: adjoint, 2: forward with the binned tally's stage
+// Instantiation name expressions, per kind and waves (2..4).  Kind 0: forward, 1: adjoint, 2:
+// forward with the binned tally's stage, 3: forward over several sets (batches)
+static constexpr int kJitKinds = 4;
 static std::string jit_name(uint32_t km, int kind, int waves) {
     return std::string("smcrt::packet_kernel<") + (kind == 1 ? "true" : "false") + ", " + std::to_string(km) + "u, " +
-           std::to_string(waves) + ", smcrt::SceneSweep, " + (kind == 2 ? "true" : "false") + ">";
+           std::to_string(waves) + (kind == 3 ? ", smcrt::SceneSweepSets, " : ", smcrt::SceneSweep, ") +
+           (kind == 2 ? "true" : "false") + ">";
 }
 
 static std::string jit_source(const smcrt_scene* s, uint32_t km) {
@@ -1053,6 +1057,7 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     "    static constexpr bool kDetectors = " + std::string(s->ddets.empty() ? "false" : "true") + ";\n"
                     "    static constexpr bool kWeight = false;\n"
                     "    static constexpr int kSource = " + std::to_string(s->src.kind) + ";\n"
+                    "    static constexpr bool kSets = false;\n"
                     "    template <uint32_t KM>\n"
                     "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel) {\n"
                     "        switch (r) {\n";
@@ -1112,12 +1117,12 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
         o += "            return " + (vs.empty() ? std::string("0.f") : vs.back()) + ";\n        }\n";
     }
     o += "        }\n        return 0.f;\n    }\n";
-    {  // the scene's fixed launch parameters, exactly as fill_common sets them
+    std::string bake;  // the scene's fixed launch parameters, exactly as fill_common sets them
+    {
         KParams P;
         KCold Cc;
         fill_common(s, P, Cc);
-        o += "    static SMCRT_DEV void bake(KParams& P) {\n";
-        o += "        P.nsets = 1;  // compiled kernels run single-set launches only\n";
+        std::string& o = bake;
         o += "        P.nreg = " + std::to_string(P.nreg) + ";\n";
         o += "        P.nx = " + std::to_string(P.nx) + "; P.ny = " + std::to_string(P.ny) + "; P.nz = " + std::to_string(P.nz) +
              ";\n";
@@ -1128,10 +1133,15 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
             jit_float(o, f.second);
             o += ";\n";
         }
-        o += "    }\n";
     }
-    o += "};\n";
-    for (int kind = 0; kind < 3; ++kind)
+    o += "    static SMCRT_DEV void bake(KParams& P) {\n"
+         "        P.nsets = 1;  // single-set launches\n" + bake + "    }\n};\n";
+    // the same program for batched launches: any number of sets of this scene (set lookup, per-lane
+    // outcome counters)
+    o += "struct SceneSweepSets : SceneSweep {\n"
+         "    static constexpr bool kSets = true;\n"
+         "    static SMCRT_DEV void bake(KParams& P) {\n" + bake + "    }\n};\n";
+    for (int kind = 0; kind < kJitKinds; ++kind)
         for (int w = 2; w <= 4; ++w) o += "template __global__ void " + jit_name(km, kind, w).substr(7) + "(KParams);\n";
     o += "}  // namespace smcrt\n";
     return o;
@@ -1159,8 +1169,8 @@ static std::string jit_cache_dir() {
 }
 
 // cache file: u32 count, then per name (u32 length, bytes), then u64 code size and the code
-// kernels in a compiled module: forward and adjoint, each built for 2, 3 and 4 waves
-static constexpr uint32_t kJitKernels = 9;
+// kernels in a compiled module: every kind, each built for 2, 3 and 4 waves
+static constexpr uint32_t kJitKernels = 3 * kJitKinds;
 
 static bool jit_cache_read(const std::string& path, std::vector<std::string>& names, std::vector<char>& code) {
     FILE* f = fopen(path.c_str(), "rb");
@@ -1271,7 +1281,7 @@ static bool jit_build(smcrt_scene* s) {
             return false;
         }
         std::vector<std::string> exprs;
-        for (int kind = 0; kind < 3; ++kind)
+        for (int kind = 0; kind < kJitKinds; ++kind)
             for (int w = 2; w <= 4; ++w) {
                 exprs.push_back(jit_name(km, kind, w));
                 (void)hiprtcAddNameExpression(prog, exprs.back().c_str());
@@ -1311,8 +1321,8 @@ static bool jit_build(smcrt_scene* s) {
         J.mod = nullptr;
         return false;
     }
-    hipFunction_t* fn[kJitKernels] = {&J.fwd[0], &J.fwd[1], &J.fwd[2], &J.adj[0], &J.adj[1], &J.adj[2],
-                                      &J.bin[0], &J.bin[1], &J.bin[2]};
+    hipFunction_t* fn[kJitKernels] = {&J.fwd[0],  &J.fwd[1],  &J.fwd[2],  &J.adj[0], &J.adj[1], &J.adj[2],
+                                      &J.bin[0],  &J.bin[1],  &J.bin[2],  &J.sets[0], &J.sets[1], &J.sets[2]};
     if (names.size() != kJitKernels) {
         J.error = "unexpected kernel count in the compiled module";
         (void)hipModuleUnload(J.mod);
@@ -1333,26 +1343,39 @@ static bool jit_build(smcrt_scene* s) {
 
 // Run-time compiled kernels for this launch?  SMCRT_JIT=0 never, =1 always; by default for
 // launches of at least 2^22 packets (a compile costs a few seconds once per program; the disk
-// cache makes later processes load it directly).  Batches always use the library's kernels, and
-// so do launches asking for the test diagnostics (event trace, scatter moments, end-early) or the
-// weight_scatter estimator, which the compiled kernels leave out.
+// cache makes later processes load it directly).  Launches asking for the test diagnostics (event
+// trace, scatter moments, end-early) or the weight_scatter estimator, which the compiled kernels
+// leave out, use the library's kernels.
+static bool jit_env(unsigned long long packets, bool* decided) {
+    const char* e = getenv("SMCRT_JIT");
+    *decided = e && (!strcmp(e, "0") || !strcmp(e, "1"));
+    if (*decided) return !strcmp(e, "1");
+    return packets >= (1ull << 22);
+}
 static bool jit_wanted(const smcrt_run_opts* o) {
     if (o && (o->trace_packets > 0 || (o->flags & (SMCRT_FLAG_MOMENTS | SMCRT_FLAG_END_EARLY | SMCRT_FLAG_WEIGHT))))
         return false;
-    const char* e = getenv("SMCRT_JIT");
-    if (e && !strcmp(e, "0")) return false;
-    if (e && !strcmp(e, "1")) return true;
-    return o && o->n_packets >= (1ull << 22);
+    bool decided;
+    const bool w = jit_env(o ? o->n_packets : 0, &decided);
+    return decided ? w : (o && w);
+}
+// The same for one geometry group of a batch (its sets' packets together).
+static bool batch_jit_wanted(const smcrt_run_opts* shape, unsigned long long packets) {
+    if (shape && (shape->flags & SMCRT_FLAG_WEIGHT)) return false;
+    bool decided;
+    return jit_env(packets, &decided);
 }
 
-static hipFunction_t jit_function(const smcrt_scene* s, bool adj, int waves, bool bin = false) {
-    return adj ? s->jit.adj[waves - 2] : (bin ? s->jit.bin[waves - 2] : s->jit.fwd[waves - 2]);
+static hipFunction_t jit_function(const smcrt_scene* s, bool adj, int waves, bool bin = false, bool sets = false) {
+    if (adj) return s->jit.adj[waves - 2];
+    return bin ? s->jit.bin[waves - 2] : (sets ? s->jit.sets[waves - 2] : s->jit.fwd[waves - 2]);
 }
 
 static hipError_t kernel_occupancy(const smcrt_scene* s, bool adj, int waves, bool compiled, int block, size_t lds,
-                                   int* active, bool bin = false) {
+                                   int* active, bool bin = false, bool sets = false) {
     if (compiled)
-        return hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(active, jit_function(s, adj, waves, bin), block, lds);
+        return hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(active, jit_function(s, adj, waves, bin, sets), block,
+                                                                  lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(active, packet_kernel_for(s, adj, waves), block, lds);
 }
 
@@ -1368,8 +1391,8 @@ static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const Lau
                                        KParams& P) {
     void* args[] = {&P};
     if (sh.compiled)
-        return hipModuleLaunchKernel(jit_function(s, adj, sh.waves, sh.bin), (unsigned)sh.blocks, 1, 1, (unsigned)sh.block, 1, 1,
-                                     (unsigned)sh.lds, stream, args, nullptr);
+        return hipModuleLaunchKernel(jit_function(s, adj, sh.waves, sh.bin, sh.sets), (unsigned)sh.blocks, 1, 1,
+                                     (unsigned)sh.block, 1, 1, (unsigned)sh.lds, stream, args, nullptr);
     return hipLaunchKernel((const void*)packet_kernel_for(s, adj, sh.waves), dim3(sh.blocks), dim3(sh.block), args,
                            sh.lds, stream);
 }
@@ -1377,10 +1400,11 @@ static hipError_t launch_packet_kernel(const smcrt_scene* s, bool adj, const Lau
 // Launch shape: block size, resident blocks, chunk, walk cap and dynamic LDS (the region-indexed
 // material tables of n_sets sets, plus the adjoint's per-lane scores and gradients).
 static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, bool adj, int n_sets, int tile_vox = 0,
-                              int adj_slots = 0, bool compiled = false, int nhot = 0) {
+                              int adj_slots = 0, bool compiled = false, int nhot = 0, bool sets = false) {
     LaunchShape L;
     L.compiled = compiled;
     L.bin = compiled && !adj && nhot > 0;
+    L.sets = compiled && !adj && !L.bin && sets;
     L.block = (o && o->block_size > 0) ? o->block_size : 256;
     int bpc = (o && o->blocks_per_cu > 0) ? o->blocks_per_cu : 4;
     L.chunk = (o && o->chunk > 0) ? o->chunk : 64;
@@ -1406,7 +1430,7 @@ static LaunchShape pick_shape(const smcrt_scene* s, const smcrt_run_opts* o, boo
     if (!(o && o->blocks_per_cu > 0)) {
         // persistent grid: as many blocks as can be resident at once
         int active = 0;
-        hipError_t e = kernel_occupancy(s, adj, L.waves, compiled, L.block, L.lds, &active, L.bin);
+        hipError_t e = kernel_occupancy(s, adj, L.waves, compiled, L.block, L.lds, &active, L.bin, L.sets);
         if (e == hipSuccess && active > 0) bpc = active;
     }
     L.blocks = s->num_cu * bpc;
@@ -2096,7 +2120,10 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
             return fail(SMCRT_ERANGE, "a geometry group holds %zu sets x %zu regions; at most %d material entries fit in LDS",
                         g.sets.size(), s->regions.size(), SMCRT_LDS_MATS);
         }
-        g.sh = pick_shape(s, shape, false, (int)g.sets.size());
+        // large groups run the scene's compiled kernel for several sets (same transport, the
+        // program as straight-line code)
+        const bool compiled = batch_jit_wanted(shape, acc) && jit_build(s);
+        g.sh = pick_shape(s, shape, false, (int)g.sets.size(), 0, 0, compiled, 0, compiled);
         P.chunk = g.sh.chunk;
         P.dda_cap = g.sh.walk_cap;
         P.refill_min = g.sh.refill_min;

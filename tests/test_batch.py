@@ -49,6 +49,30 @@ def test_batch_sets_reproduce_standalone_runs(gpu):
         np.testing.assert_allclose(jm, r.jmean, rtol=1e-9, atol=1e-13)
 
 
+def test_compiled_batch_reproduces_compiled_runs(gpu, monkeypatch, tmp_path):
+    """SMCRT_JIT=1: the geometry groups run the scenes' run-time compiled kernels for several sets
+    (set lookup and per-lane outcome counters), the standalone runs the single-set compiled
+    kernels: same packets, same transport."""
+    monkeypatch.setenv("SMCRT_JIT", "1")
+    monkeypatch.setenv("SMCRT_CACHE_DIR", str(tmp_path))
+    engines = _engines()
+    rng = np.random.default_rng(11)
+    sets = _sets(engines, rng, 8)
+    b = Batch(engines, sets, seed=77)
+    b.run()
+    for eng in engines:
+        name = eng.kernel_variant()[0]
+        assert "compiled" in name and "failed" not in name, name
+    for i, s in enumerate(sets):
+        jm, st = b.read(i)
+        eng = engines[s.scene]
+        eng.set_materials(mus=np.array(s.mus), mua=np.array(s.mua))
+        r = eng.run(s.n_packets, seed=77, first_packet=s.first_packet)
+        for k in OUT:
+            assert st[k] == r.stats[k], (i, k, st[k], r.stats[k])
+        np.testing.assert_allclose(jm, r.jmean, rtol=1e-9, atol=1e-13)
+
+
 def test_graph_replays_repeat(gpu):
     engines = _engines()
     rng = np.random.default_rng(8)

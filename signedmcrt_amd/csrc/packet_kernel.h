@@ -800,8 +800,9 @@ SMCRT_DEV bool walk_begin(const KParams& P, const Lane& L, Walk& W, float d) {
 
 // How the sweep evaluates the scene's regions.  The library's kernels interpret the SDF program
 // (region_eval); a kernel compiled at run time for one scene passes a policy whose region<KM>(r,
-// p, accel) is that region's program as straight-line code (constants as immediates, no node
-// loads or opcode dispatch, independent regions interleaved by the compiler).
+// p, accel, h) is that region's program as straight-line code (constants as immediates, no node
+// loads or opcode dispatch, independent regions interleaved by the compiler; h: the lane's
+// gathered polygon, from poly_hit<KM>(p, accel) when kGather).
 struct InterpretedSweep {
     static constexpr bool kCompiled = false;
     static constexpr int kRegions = 0;
@@ -810,11 +811,12 @@ struct InterpretedSweep {
     static constexpr bool kWeight = true;       // the weight_scatter estimator (SMCRT_FLAG_WEIGHT)
     static constexpr int kSource = -1;          // source kind (-1: read at run time)
     static constexpr bool kSets = true;         // launches may hold several sets (batches)
+    static constexpr bool kGather = false;      // gathered polygon evaluation (PolyHit)
     // a compiled policy overwrites the scene's fixed launch parameters (grid, epsilons, region
     // count) with constants
     static SMCRT_DEV void bake(KParams&) {}
     template <uint32_t KM>
-    static SMCRT_DEV float region(int, const V3<float>&, const float* __restrict__) { return 0.f; }
+    static SMCRT_DEV float region(int, const V3<float>&, const float* __restrict__, const PolyHit&) { return 0.f; }
 };
 
 // WAVES: resident waves per SIMD the registers are allocated for (2: at most 256 VGPRs, 3: at most
@@ -1020,8 +1022,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             };
             if constexpr (SW::kCompiled) {
                 // the scene's program compiled into the kernel: straight-line code per region
+                // (scenes of several polygons: each lane's own polygon evaluated first, gathered)
+                PolyHit h{-1, 0.f};
+                if constexpr (SW::kGather) h = SW::template poly_hit<KM>(p, P.accel);
 #pragma unroll
-                for (int r = 0; r < SW::kRegions; ++r) visit(r, SW::template region<KM>(r, p, P.accel));
+                for (int r = 0; r < SW::kRegions; ++r) visit(r, SW::template region<KM>(r, p, P.accel, h));
             } else {
                 for (int r = 0; r < P.nreg; ++r)
                     visit(r, region_eval<float, KM>(nodes, regions[r].first, regions[r].count, p, P.accel));

@@ -685,6 +685,14 @@ extern "C" int smcrt_scene_create(const smcrt_scene_desc* d, smcrt_scene** out) 
                     return fail(SMCRT_EINVAL, "region %d node %d: polygon accelerator failed its index check", r, i);
                 }
                 dn.p[0] = (float)pb;
+                // the accelerator's grid rectangle and margin (gx0, gy0, gx1, gy1, margin): constants
+                // of the gathered evaluation in the scene's compiled kernel
+                const float* H = s->accel.data() + pb;
+                dn.q[0] = H[2];
+                dn.q[1] = H[3];
+                dn.q[2] = H[6];
+                dn.q[3] = H[7];
+                dn.q[4] = H[8];
                 if (s->accel.size() > (1u << 24)) {
                     delete s;
                     return fail(SMCRT_ERANGE, "polygon accelerators exceed 2^24 floats");
@@ -1048,7 +1056,53 @@ static std::string jit_name(uint32_t km, int kind, int waves) {
            (kind == 2 ? "true" : "false") + ">";
 }
 
+// The statements computing node i's input position inside region R (the modifiers open at node
+// i, then its own transform), with `pre` prefixed to the temporaries' names; returns the
+// position's name.
+static std::string jit_pos_chain(std::string& o, const smcrt_scene* s, const DevRegion& R, int i, const std::string& pre) {
+    std::vector<int> open;
+    for (int k = 0; k < i; ++k) {
+        const DevNode& n = s->nodes[R.first + k];
+        if (n.op == SMCRT_OP_MOD_BEGIN) open.push_back(k);
+        else if (n.op == SMCRT_OP_MOD_END && !open.empty()) open.pop_back();
+    }
+    std::string q = "p";
+    for (int k : open) {
+        const DevNode& n = s->nodes[R.first + k];
+        const std::string id = pre + std::to_string(k);
+        jit_array(o, "P" + id, n.p, 8);
+        o += "            const V3<float> x" + id + " = mod_pos<float, KM>(" + std::to_string(n.kind) + ", P" + id + ", " +
+             q + ");\n";
+        q = "x" + id;
+    }
+    const DevNode& n = s->nodes[R.first + i];
+    const std::string id = pre + std::to_string(i);
+    if (n.ident == 0) {
+        jit_array(o, "T" + id, n.t, 12);
+        o += "            const V3<float> x" + id + " = xform<float>(T" + id + ", " + q + ");\n";
+        q = "x" + id;
+    } else if (n.ident == 2) {  // translation only
+        o += "            const V3<float> x" + id + "{" + q + ".x + ";
+        jit_float(o, n.t[9]);
+        o += ", " + q + ".y + ";
+        jit_float(o, n.t[10]);
+        o += ", " + q + ".z + ";
+        jit_float(o, n.t[11]);
+        o += "};\n";
+        q = "x" + id;
+    }
+    return q;
+}
+
 static std::string jit_source(const smcrt_scene* s, uint32_t km) {
+    // polygon primitives, numbered in program order (the gathered evaluation's indices)
+    std::vector<std::pair<int, int>> polys;  // (region, node)
+    for (size_t r = 0; r < s->regions.size(); ++r)
+        for (int i = 0; i < s->regions[r].count; ++i) {
+            const DevNode& n = s->nodes[s->regions[r].first + i];
+            if (n.op == SMCRT_OP_PRIM && n.kind == SMCRT_POLYGON) polys.push_back({(int)r, i});
+        }
+    const bool gather = polys.size() >= 2;
     std::string o = "#include \"smcrt.h\"\n#include \"packet_kernel.h\"\nnamespace smcrt {\n"
                     "struct SceneSweep {\n"
                     "    static constexpr bool kCompiled = true;\n"
@@ -1058,9 +1112,34 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     "    static constexpr bool kWeight = false;\n"
                     "    static constexpr int kSource = " + std::to_string(s->src.kind) + ";\n"
                     "    static constexpr bool kSets = false;\n"
-                    "    template <uint32_t KM>\n"
-                    "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel) {\n"
-                    "        switch (r) {\n";
+                    "    static constexpr bool kGather = " + std::string(gather ? "true" : "false") + ";\n";
+    if (gather) {
+        // poly_hit: the first polygon whose grid rectangle holds the lane's position, evaluated
+        o += "    template <uint32_t KM>\n"
+             "    static SMCRT_DEV PolyHit poly_hit(const V3<float>& p, const float* __restrict__ accel) {\n"
+             "        int k = -1, base = 0;\n"
+             "        V3<float> xs = p;\n";
+        for (size_t j = 0; j < polys.size(); ++j) {
+            const DevRegion& R = s->regions[polys[j].first];
+            const DevNode& n = s->nodes[R.first + polys[j].second];
+            o += "        {\n";
+            const std::string q = jit_pos_chain(o, s, R, polys[j].second, "");
+            jit_array(o, "Q", n.q, 8);
+            o += "            const bool t = (k < 0) & polygon_rect_inside(Q, " + q + ");\n"
+                 "            k = t ? " + std::to_string(j) + " : k;\n"
+                 "            base = t ? " + std::to_string((int)n.p[0]) + " : base;\n"
+                 "            xs = t ? " + q + " : xs;\n"
+                 "        }\n";
+        }
+        o += "        PolyHit h{k, 0.f};\n"
+             "        if (k >= 0) h.d = polygon_eval<float>(accel, base, xs);\n"
+             "        return h;\n"
+             "    }\n";
+    }
+    o += "    template <uint32_t KM>\n"
+         "    static SMCRT_DEV float region(int r, const V3<float>& p, const float* __restrict__ accel, const PolyHit& h) {\n"
+         "        switch (r) {\n";
+    size_t pj = 0;  // next polygon index
     for (size_t r = 0; r < s->regions.size(); ++r) {
         const DevRegion& R = s->regions[r];
         o += "        case " + std::to_string(r) + ": {\n";
@@ -1086,8 +1165,12 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     o += "};\n";
                     q = "x" + id;
                 }
-                o += "            const float v" + id + " = prim_eval<float, KM>(" + kind + ", P" + id + ", Q" + id + ", " +
-                     q + ", accel);\n";
+                if (gather && n.kind == SMCRT_POLYGON)
+                    o += "            const float v" + id + " = poly_value(" + std::to_string(pj++) + ", " +
+                         std::to_string((int)n.p[0]) + ", Q" + id + ", " + q + ", accel, h);\n";
+                else
+                    o += "            const float v" + id + " = prim_eval<float, KM>(" + kind + ", P" + id + ", Q" + id + ", " +
+                         q + ", accel);\n";
                 vs.push_back("v" + id);
             } else if (n.op == SMCRT_OP_CSG) {
                 jit_array(o, "Q" + id, n.q, 8);

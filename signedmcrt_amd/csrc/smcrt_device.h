@@ -264,6 +264,38 @@ SMCRT_DEV T polygon_eval(const float* __restrict__ A, int base, const V3<T>& p) 
     return g_neg_if(g_sqrt(best), inside);
 }
 
+// Gathered polygon evaluation for kernels compiled for one scene.  A point lies inside few
+// polygons' grid rectangles (one, where rectangles do not overlap); every other polygon's value is
+// the rectangle distance plus the margin, arithmetic on constants.  So the compiled sweep first
+// finds, per lane, the first polygon (in program order) whose rectangle holds its position and
+// evaluates that one through its accelerator (PolyHit), all lanes at once; a polygon's region
+// code then takes the lane's gathered value, the rectangle formula, or (a lane inside a second
+// rectangle, a wave-level branch no lane takes where rectangles are disjoint) a full evaluation.
+// The values are those of polygon_eval.  Q (the node's, host-filled): gx0, gy0, gx1, gy1, margin.
+struct PolyHit {
+    int k;    // the lane's gathered polygon (-1: inside none)
+    float d;  // its value
+};
+SMCRT_DEV bool polygon_rect_inside(const float* __restrict__ Q, const V3<float>& p) {
+    return (p.x >= Q[0]) & (p.x < Q[2]) & (p.y >= Q[1]) & (p.y < Q[3]);
+}
+SMCRT_DEV float polygon_outside(const float* __restrict__ Q, const V3<float>& p) {
+    const float dx = fmaxf(fmaxf(Q[0] - p.x, p.x - Q[2]), 0.f);
+    const float dy = fmaxf(fmaxf(Q[1] - p.y, p.y - Q[3]), 0.f);
+    return f_sqrt(dx * dx + dy * dy) + Q[4];
+}
+SMCRT_DEV float poly_value(int j, int base, const float* __restrict__ Q, const V3<float>& p,
+                           const float* __restrict__ A, const PolyHit& h) {
+    if (h.k == j) return h.d;
+    const bool in = polygon_rect_inside(Q, p);
+    float v = polygon_outside(Q, p);
+    if (__builtin_amdgcn_ballot_w64(in)) {
+        rare_path();
+        if (in) v = polygon_eval<float>(A, base, p);
+    }
+    return v;
+}
+
 // ------------------------------------------------------------------------------------------
 // Kind masks.  The packet kernel is compiled for a few sets of primitive / operator / modifier
 // kinds (the host launches the smallest variant covering the scene): the interpreter's code is a

@@ -52,10 +52,32 @@ def _zoo_scene():
     return Scene(arr, cart_grid(32, 32, 32, 2.0, 2.0, 2.0), source("point", (0.05, 0.13, 0.21)))
 
 
-@pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels", "polygon_regions", "zoo"])
+def _nested_polygons_scene(square_first: bool):
+    """An L-shaped polygon and a square in its notch, extruded in a slab: disjoint polygons whose
+    accelerator rectangles overlap, so a point in the square lies inside both rectangles (the
+    compiled kernels' gathered polygon evaluation takes the first, the second through its full
+    evaluation).  Program order: the L first, or the square first."""
+    from signedmcrt_amd.sdf import extrude, polygon
+    ell = np.array([(-1.0, -1.0), (1.0, -1.0), (1.0, 0.0), (0.0, 0.0), (0.0, 1.0), (-1.0, 1.0)])
+    sq = np.array([(0.2, 0.2), (0.8, 0.2), (0.8, 0.8), (0.2, 0.8)])
+    polys = [(ell, 0.3), (sq, 1.5)]
+    if square_first:
+        polys = polys[::-1]
+    arr = [extrude(polygon(v, mono(50.0, mua, 0.85, 1.4), k + 1), 0.25) for k, (v, mua) in enumerate(polys)]
+    arr.append(box((2.4, 2.4, 0.5), mono(50.0, 0.2, 0.85, 1.4), 3))
+    arr.append(box((4.4, 4.4, 2.5), mono(0.0, 0.0, 0.0, 1.0), 4))
+    src = source("uniform", direction=(0.0, 0.0, -1.0), point1=(-1.2, -1.2, 0.25 - 1e-4),
+                 point2=(2.4, 0.0, 0.0), point3=(0.0, 2.4, 0.0))
+    return Scene(arr, cart_grid(40, 40, 8, 1.2, 1.2, 0.25), src)
+
+
+@pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels", "polygon_regions", "zoo",
+                                  "nested_polygons", "nested_polygons_square_first"])
 def test_compiled_kernels_match_interpreter(gpu, name, monkeypatch, tmp_path):
     if name == "zoo":
         sc = _zoo_scene()
+    elif name.startswith("nested_polygons"):
+        sc = _nested_polygons_scene(name.endswith("square_first"))
     else:
         kw = {"nxyz": 32} if name == "skin_vessels" else ({"n_side": 2, "n_edges": 40, "nxy": 40, "nz": 8}
                                                           if name == "polygon_regions" else {})

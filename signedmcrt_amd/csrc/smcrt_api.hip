@@ -1103,6 +1103,25 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
             if (n.op == SMCRT_OP_PRIM && n.kind == SMCRT_POLYGON) polys.push_back({(int)r, i});
         }
     const bool gather = polys.size() >= 2;
+    // pairwise disjoint grid rectangles in one shared position frame (the same modifiers and
+    // transform before every polygon): a lane lies inside at most one rectangle
+    bool disjoint = gather;
+    if (gather) {
+        std::string frame0;
+        for (size_t j = 0; j < polys.size() && disjoint; ++j) {
+            std::string f;
+            (void)jit_pos_chain(f, s, s->regions[polys[j].first], polys[j].second, "");
+            if (j == 0) frame0 = f;
+            else disjoint = f == frame0;
+        }
+        for (size_t a = 0; a < polys.size() && disjoint; ++a)
+            for (size_t b = a + 1; b < polys.size() && disjoint; ++b) {
+                const float* qa = s->nodes[s->regions[polys[a].first].first + polys[a].second].q;
+                const float* qb = s->nodes[s->regions[polys[b].first].first + polys[b].second].q;
+                // [gx0, gx1) x [gy0, gy1) rectangles
+                disjoint = qa[2] <= qb[0] || qb[2] <= qa[0] || qa[3] <= qb[1] || qb[3] <= qa[1];
+            }
+    }
     std::string o = "#include \"smcrt.h\"\n#include \"packet_kernel.h\"\nnamespace smcrt {\n"
                     "struct SceneSweep {\n"
                     "    static constexpr bool kCompiled = true;\n"
@@ -1165,7 +1184,10 @@ static std::string jit_source(const smcrt_scene* s, uint32_t km) {
                     o += "};\n";
                     q = "x" + id;
                 }
-                if (gather && n.kind == SMCRT_POLYGON)
+                if (gather && n.kind == SMCRT_POLYGON && disjoint)
+                    o += "            const float v" + id + " = poly_value_disjoint(" + std::to_string(pj++) + ", Q" + id +
+                         ", " + q + ", h);\n";
+                else if (gather && n.kind == SMCRT_POLYGON)
                     o += "            const float v" + id + " = poly_value(" + std::to_string(pj++) + ", " +
                          std::to_string((int)n.p[0]) + ", Q" + id + ", " + q + ", accel, h);\n";
                 else

@@ -284,6 +284,11 @@ SMCRT_DEV float polygon_outside(const float* __restrict__ Q, const V3<float>& p)
     const float dy = fmaxf(fmaxf(Q[1] - p.y, p.y - Q[3]), 0.f);
     return f_sqrt(dx * dx + dy * dy) + Q[4];
 }
+// Scenes whose polygons share one position frame and have pairwise disjoint rectangles (checked on
+// the host): a lane inside polygon j's rectangle gathered polygon j.
+SMCRT_DEV float poly_value_disjoint(int j, const float* __restrict__ Q, const V3<float>& p, const PolyHit& h) {
+    return h.k == j ? h.d : polygon_outside(Q, p);
+}
 SMCRT_DEV float poly_value(int j, int base, const float* __restrict__ Q, const V3<float>& p,
                            const float* __restrict__ A, const PolyHit& h) {
     if (h.k == j) return h.d;

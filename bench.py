@@ -380,6 +380,12 @@ def batched_sets(n_sets: int, seed: int = 20240917):
     return out
 
 
+def batch_shape(shape):
+    """The launch-shape knobs of a tuned shape that a batch takes: the schedule only (batches
+    tally straight to HBM: no LDS tile, no binned tally; no adjoint)."""
+    return {k: v for k, v in shape.items() if k in ("block", "chunk", "walk_cap", "refill_min", "waves")}
+
+
 def bench_batched(args):
     """One HIP-graph replay = every set's packets in one persistent launch per geometry."""
     import numpy as np
@@ -407,7 +413,7 @@ def bench_batched(args):
     tuned = None
     if not args.no_tune:
         res = engines[1].tune(use_cache=False, adjoint=False)
-        shape = {k: v for k, v in res.shape.items() if k != "tile"}  # batches tally straight to HBM
+        shape = batch_shape(res.shape)
         tuned = {"shape": res.shape, "tuned_on": "refracting_sphere 64^3", "speedup_vs_default": res.speedup}
     b = Batch(engines, sets, seed=123456789, **shape)
     for _ in range(args.warmup):

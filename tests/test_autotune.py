@@ -55,6 +55,22 @@ def test_run_kwargs_maps_tile_to_flag():
     assert autotune.run_kwargs(autotune.DEFAULT)["flags"] == 0
 
 
+def test_tuned_shape_runs_a_batch():
+    """bench.py's batched workload hands the tuner's shape to Batch, which builds its launch
+    options with engine.run_opts: every knob the tuner returns (adjoint waves included) must map
+    to an accepted keyword after batch_shape."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    from signedmcrt_amd.engine import run_opts
+    shape = {k: v[-1] for k, v in autotune.SPACE.items()}
+    shape["adj_waves"] = 4
+    o = run_opts(0, **bench.batch_shape(shape))
+    assert o.block_size == shape["block"] and o.chunk == shape["chunk"] and o.walk_cap == shape["walk_cap"]
+    assert o.refill_min == shape["refill_min"] and o.waves == shape["waves"]
+
+
 def test_fingerprint_tracks_the_scene():
     a, _ = scenes.build("slab")
     b, _ = scenes.build("slab")

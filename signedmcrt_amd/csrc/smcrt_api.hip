@@ -116,8 +116,8 @@ struct BinWork {
 };
 // Sum a work item's entries in an FP64 LDS tile of the bin's voxels, then add the tile to the
 // grid (plain read-modify-write when the item holds the whole bin, atomics otherwise).  The
-// item's page ids and fills are read into LDS first; each thread then keeps 8 entry loads in
-// flight (the loads are independent, so the pages stream at memory speed).
+// item's page ids and fills are read into LDS first; each wavefront then streams whole pages,
+// two at a time (sixteen independent 8-byte loads in flight per lane).
 constexpr int BIN_VOX = 1 << SMCRT_BIN_SHIFT;
 constexpr uint32_t BIN_PAGES_PER_ITEM = 2048;  // reduction work item: 1M entries
 __global__ void __launch_bounds__(1024) k_bin_accum(const BinWork* __restrict__ work, const uint32_t* __restrict__ list,
@@ -134,21 +134,29 @@ __global__ void __launch_bounds__(1024) k_bin_accum(const BinWork* __restrict__ 
         pfill[i] = page_fill[p];
     }
     __syncthreads();
-    const uint32_t n = np * SMCRT_BIN_PAGE;
-    constexpr int U = 8;
-    for (uint32_t j0 = threadIdx.x; j0 < n; j0 += U * blockDim.x) {
-        uint2 e[U];
-        bool ok[U];
+    // each wavefront streams two whole pages at a time: page id and fill read once per page,
+    // sixteen entry loads in flight per lane
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (uint32_t k0 = 2u * wid; k0 < np; k0 += 2u * nw) {
+        uint2 e[16];
+        uint32_t f[2];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t j = j0 + (uint32_t)u * blockDim.x;
-            const uint32_t k = j < n ? j / SMCRT_BIN_PAGE : 0u, i = j % SMCRT_BIN_PAGE;
-            ok[u] = j < n && i < pfill[k];
-            e[u] = ok[u] ? pool[(size_t)pid[k] * SMCRT_BIN_PAGE + i] : make_uint2(0u, 0u);
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t k = k0 + (uint32_t)q;
+            f[q] = k < np ? pfill[k] : 0u;
+            const uint2* src = pool + (size_t)(k < np ? pid[k] : 0u) * SMCRT_BIN_PAGE;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t i = lane + 64u * (uint32_t)u;
+                e[8 * q + u] = i < f[q] ? src[i] : make_uint2(0u, 0u);
+            }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (ok[u]) atomicAdd(t + (e[u].x & (BIN_VOX - 1)), (double)__uint_as_float(e[u].y));
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (lane + 64u * (uint32_t)u < f[q])
+                    atomicAdd(t + (e[8 * q + u].x & (BIN_VOX - 1)), (double)__uint_as_float(e[8 * q + u].y));
     }
     __syncthreads();
     const size_t g0 = (size_t)w.bin << SMCRT_BIN_SHIFT;

@@ -97,14 +97,17 @@ def atomic_ceiling(nv, n=1 << 26, reps=3):
     return n / (best * 1e-3)
 
 
+JIT_MIN_PACKETS = 1 << 19  # the library's threshold for its run-time compiled kernels
+
+
 def kernel_label(eng, waves, n_packets):
     """The forward kernel a launch of n_packets ran, named as rocprof reports it: the library's
     variant, or the kernel compiled at run time for the scene's program (instantiated on the
-    scene's exact kinds), which launches of >= 2^22 packets use once it is built (SMCRT_JIT=1:
+    scene's exact kinds), which launches of >= JIT_MIN_PACKETS use once it is built (SMCRT_JIT=1:
     every launch, =0: none)."""
     name, scene_mask, variant_mask = eng.kernel_variant()
     jit = os.environ.get("SMCRT_JIT", "")
-    if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= (1 << 22))):
+    if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= JIT_MIN_PACKETS)):
         # the last BIN parameter: the binned tally's stage (the library reports the last launch's)
         return "packet_kernel<false, %du, %d, SceneSweep, %s> (%s)" % (
             scene_mask, waves, "true" if "+binned" in name else "false", name)
@@ -113,11 +116,11 @@ def kernel_label(eng, waves, n_packets):
 
 def batch_kernel_label(eng, waves, n_packets):
     """The kernel a batch's geometry group of n_packets ran: the scene's run-time compiled kernel
-    for several sets (groups of >= 2^22 packets, once built; SMCRT_JIT=1: every group, =0: none)
+    for several sets (groups of >= JIT_MIN_PACKETS, once built; SMCRT_JIT=1: every group, =0: none)
     or the library's variant."""
     name, scene_mask, variant_mask = eng.kernel_variant()
     jit = os.environ.get("SMCRT_JIT", "")
-    if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= (1 << 22))):
+    if "compiled" in name and "failed" not in name and (jit == "1" or (jit != "0" and n_packets >= JIT_MIN_PACKETS)):
         return "packet_kernel<false, %du, %d, SceneSweepSets, false>" % (scene_mask, waves)
     return "packet_kernel<false, %du, %d, InterpretedSweep, false>" % (variant_mask, min(waves, 3))
 

@@ -1455,15 +1455,17 @@ static bool jit_build(smcrt_scene* s) {
 }
 
 // Run-time compiled kernels for this launch?  SMCRT_JIT=0 never, =1 always; by default for
-// launches of at least 2^22 packets (a compile costs a few seconds once per program; the disk
-// cache makes later processes load it directly).  Launches asking for the test diagnostics (event
+// launches of at least kJitMinPackets = 2^19 packets (a compile costs a few seconds once per
+// program; the disk cache makes later processes load it directly; a 1e6-packet slab launch runs
+// 2.2x faster compiled).  Launches asking for the test diagnostics (event
 // trace, scatter moments, end-early) or the weight_scatter estimator, which the compiled kernels
 // leave out, use the library's kernels.
+static constexpr unsigned long long kJitMinPackets = 1ull << 19;
 static bool jit_env(unsigned long long packets, bool* decided) {
     const char* e = getenv("SMCRT_JIT");
     *decided = e && (!strcmp(e, "0") || !strcmp(e, "1"));
     if (*decided) return !strcmp(e, "1");
-    return packets >= (1ull << 22);
+    return packets >= kJitMinPackets;
 }
 static bool jit_wanted(const smcrt_run_opts* o) {
     if (o && (o->trace_packets > 0 || (o->flags & (SMCRT_FLAG_MOMENTS | SMCRT_FLAG_END_EARLY | SMCRT_FLAG_WEIGHT))))

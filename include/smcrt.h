@@ -211,6 +211,18 @@ enum {
                                        summation order                                          */
 };
 
+/* Partial grids per XCD (flags bits 8-10 hold log2(copies per XCD) + 1; 0 -> the library's
+   default: 1 per XCD for single-set launches, 8 for batches, whose chords have no LDS tile).  The
+   tally's global FP64 atomics spread over the copies, so hot voxels contend less; each copy costs
+   one grid of device memory and a clear and a sum per launch.  Up to 8 GiB of copies by default
+   and 40 GiB when more than one per XCD is asked for, within half the free device memory.
+   SMCRT_FLAG_GRID_COPIES(k): 2^k partial grids per XCD, k = 0..6.  Same result up to FP64
+   summation order. */
+#define SMCRT_FLAG_GRID_COPIES_SHIFT 8
+#define SMCRT_FLAG_GRID_COPIES_MASK (7u << SMCRT_FLAG_GRID_COPIES_SHIFT)
+#define SMCRT_FLAG_GRID_COPIES(k) ((uint32_t)((k) + 1) << SMCRT_FLAG_GRID_COPIES_SHIFT)
+
+
 typedef struct smcrt_run_opts {
     uint64_t n_packets;    /* packets to launch (`nphotons`)                                 */
     uint64_t seed;         /* RNG key (`iseed`)                                              */

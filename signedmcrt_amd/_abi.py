@@ -32,6 +32,15 @@ FLAG_WEIGHT = 1 << 4
 FLAG_SHARED_GRID = 1 << 5
 FLAG_DIRECT_TALLY = 1 << 6
 FLAG_BINNED_TALLY = 1 << 7
+FLAG_GRID_COPIES_SHIFT = 8  # flags bits 8-10: log2(partial grids per XCD) + 1, 0 = library default
+
+
+def flag_grid_copies(per_xcd: int) -> int:
+    """The flags field asking for `per_xcd` (a power of two, 1..64) partial grids per XCD."""
+    k = int(per_xcd).bit_length() - 1
+    if per_xcd < 1 or (1 << k) != per_xcd or k > 6:
+        raise ValueError("partial grids per XCD: a power of two from 1 to 64")
+    return (k + 1) << FLAG_GRID_COPIES_SHIFT
 
 MAX_REGIONS = 64
 MAX_NODES = 4096

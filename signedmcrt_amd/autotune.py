@@ -12,6 +12,9 @@ Knobs (`SPACE`):
   tally       where chords outside the tile go: "direct" (FP64 atomics), "binned" (staged per grid
               bin in LDS, written to device pages, summed per bin after the launch) or "auto"
               (the library's choice from the scene's pilot run)
+  copies      FP64 partial grids per XCD the tally's global atomics spread over (1, 2, 4, 8):
+              more copies, less contention on hot voxels, one grid of memory and a clear and a
+              sum per launch each
   waves       resident wavefronts per SIMD the kernel is built for (2, 3 or 4: fewer registers;
               4 only for the run-time compiled kernels, the library's run 3 instead)
 
@@ -44,11 +47,13 @@ SPACE: Dict[str, Tuple] = {
     "block": (64, 128, 256),
     "tile": (True, False),
     "tally": ("auto", "direct", "binned"),
+    "copies": (1, 2, 4, 8),
     "chunk": (64, 128, 256, 512),
     "walk_cap": (1, 2, 4, 8, 16),
     "refill_min": (1, 8, 16, 32),
 }
-DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "tally": "auto", "waves": 3}
+DEFAULT = {"block": 256, "chunk": 64, "walk_cap": 4, "refill_min": 1, "tile": True, "tally": "auto", "copies": 1,
+           "waves": 3}
 # counters that must not change with the launch shape
 INVARIANT = ("launched", "absorbed", "escaped_r", "escaped_t", "killed_surface", "killed_bounce",
              "scatters", "ev_steps", "ev_crossings", "ev_interfaces")
@@ -59,6 +64,8 @@ def run_kwargs(shape: Dict) -> Dict:
     kw = {k: int(shape[k]) for k in ("block", "chunk", "walk_cap", "refill_min", "waves") if k in shape}
     kw["flags"] = (0 if shape.get("tile", True) else A.FLAG_NO_LDS_TILE) | \
         {"auto": 0, "direct": A.FLAG_DIRECT_TALLY, "binned": A.FLAG_BINNED_TALLY}[shape.get("tally", "auto")]
+    if int(shape.get("copies", 1)) != 1:  # 1 per XCD is the library's default for one set
+        kw["flags"] |= A.flag_grid_copies(int(shape["copies"]))
     return kw
 
 

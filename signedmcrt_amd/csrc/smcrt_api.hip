@@ -178,14 +178,28 @@ __global__ void k_reduce_copies(double* __restrict__ dst, const double* __restri
     }
 }
 
-// Partial grids per XCD for a tally of n doubles: how many (at least 2, else 0 = tally straight
-// into the grid), within a memory cap.
-static int xgrid_copies(int n_xcc, size_t n, uint32_t flags) {
+// Partial grids for a tally of n doubles: how many (at least 2, else 0 = tally straight into the
+// grid).  Copies per XCD from the flags' SMCRT_FLAG_GRID_COPIES field, else `per_xcd`; within 8
+// GiB for one per XCD and 40 GiB for more, and within half of `avail` (the device's free memory
+// plus what the copies already hold); whole multiples of the XCD count where more than one fits
+// (block b tallies into copy b % copies, and blocks go to the XCDs round-robin).
+static int xgrid_copies(int n_xcc, size_t n, uint32_t flags, int per_xcd, size_t avail) {
     if ((flags & SMCRT_FLAG_SHARED_GRID) || n_xcc < 2 || n == 0) return 0;
-    const size_t cap = (size_t)8 << 30;  // bytes
-    size_t c = cap / (n * sizeof(double));
-    if (c > (size_t)n_xcc) c = (size_t)n_xcc;
+    const uint32_t f = (flags & SMCRT_FLAG_GRID_COPIES_MASK) >> SMCRT_FLAG_GRID_COPIES_SHIFT;
+    const size_t per = f ? (size_t)1 << (f - 1) : (size_t)(per_xcd > 1 ? per_xcd : 1);
+    size_t cap = per > 1 ? (size_t)40 << 30 : (size_t)8 << 30;  // bytes
+    if (per > 1 && cap > avail / 2) cap = std::max(avail / 2, (size_t)8 << 30);
+    size_t c = std::min(cap / (n * sizeof(double)), (size_t)n_xcc * per);
+    if (c > (size_t)n_xcc) c -= c % (size_t)n_xcc;
     return c >= 2 ? (int)c : 0;
+}
+
+// Device memory a set of partial grids may use: free memory plus the `held` bytes their buffer
+// already holds.
+static size_t xgrid_avail(size_t held) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+    return fr + held;
 }
 
 // SDF values / exact gradients of one region at n points (one point per thread; the region's
@@ -1802,7 +1816,7 @@ static int launch_run(smcrt_scene* s, const smcrt_run_opts* o, double* d_jmean, 
     P.xstride = 0;
     P.xcopies = 0;
     const size_t nvox = (size_t)s->grid.nx * s->grid.ny * s->grid.nz;
-    const int copies = d_jmean ? xgrid_copies(s->n_xcc, nvox, o->flags) : 0;
+    const int copies = d_jmean ? xgrid_copies(s->n_xcc, nvox, o->flags, 1, xgrid_avail(s->d_xgrid.bytes)) : 0;
     if (copies) {
         int rc = s->d_xgrid.ensure((size_t)copies * nvox * sizeof(double));
         if (rc) return rc;
@@ -2182,8 +2196,10 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
     }
     b->total_doubles = ndoubles;
     b->total_stats = nstats;
-    b->copies = xgrid_copies(scenes[0]->n_xcc, ndoubles, shape ? shape->flags : 0u);
+    // 8 partial grids per XCD by default: a batch's chords have no LDS tile, so its hot voxels
+    // contend (256-set workload: 5.1 s per replay with one per XCD, 3.9 s with eight)
     HIPCHECK(hipSetDevice(b->device));
+    b->copies = xgrid_copies(scenes[0]->n_xcc, ndoubles, shape ? shape->flags : 0u, 8, xgrid_avail(b->d_xgrid.bytes));
     int rc;
     if ((rc = b->d_mats.ensure(nmats * sizeof(DevMat))) || (rc = b->d_grids.ensure(ndoubles * sizeof(double))) ||
         (rc = b->d_stats.ensure(nstats * sizeof(unsigned long long))) ||

@@ -37,7 +37,8 @@ module smcrt
     integer(c_int32_t), parameter :: SMCRT_FLAG_MOMENTS = 1, SMCRT_FLAG_END_EARLY = 2, &
                                      SMCRT_FLAG_NO_TALLY = 4, SMCRT_FLAG_NO_LDS_TILE = 8, &
                                      SMCRT_FLAG_WEIGHT = 16, SMCRT_FLAG_SHARED_GRID = 32, &
-                                     SMCRT_FLAG_DIRECT_TALLY = 64, SMCRT_FLAG_BINNED_TALLY = 128
+                                     SMCRT_FLAG_DIRECT_TALLY = 64, SMCRT_FLAG_BINNED_TALLY = 128, &
+                                     SMCRT_FLAG_GRID_COPIES_SHIFT = 8
 
     type, bind(C) :: smcrt_node
         integer(c_int32_t) :: op = 0, kind = 0

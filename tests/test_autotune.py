@@ -55,6 +55,14 @@ def test_run_kwargs_maps_tile_to_flag():
     assert autotune.run_kwargs(autotune.DEFAULT)["flags"] == 0
 
 
+def test_run_kwargs_maps_grid_copies_to_flag():
+    assert autotune.run_kwargs({"copies": 1})["flags"] == 0
+    assert autotune.run_kwargs({"copies": 8})["flags"] == 4 << A.FLAG_GRID_COPIES_SHIFT
+    assert autotune.run_kwargs({"copies": 2, "tally": "direct"})["flags"] == (2 << A.FLAG_GRID_COPIES_SHIFT) | A.FLAG_DIRECT_TALLY
+    with pytest.raises(ValueError):
+        A.flag_grid_copies(3)
+
+
 def test_tuned_shape_runs_a_batch():
     """bench.py's batched workload hands the tuner's shape to Batch, which builds its launch
     options with engine.run_opts: every knob the tuner returns (adjoint waves included) must map

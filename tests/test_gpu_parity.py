@@ -344,9 +344,9 @@ def test_kernel_variants_agree(gpu, name, monkeypatch):
 @pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels"])
 def test_per_xcd_partial_grids_match_shared_grid(gpu, name):
     """One FP64 partial grid per XCD (the default), summed after the launch, against every block
-    tallying into the one grid (SMCRT_FLAG_SHARED_GRID): same packets, same counters, the same
-    fluence to FP64 summation order; a second launch starts from cleared partial
-    grids (no carry-over)."""
+    tallying into the one grid (SMCRT_FLAG_SHARED_GRID) and four partial grids per XCD: same
+    packets, same counters, the same fluence to FP64 summation order; a second launch starts from
+    cleared partial grids (no carry-over)."""
     from signedmcrt_amd import _abi as A
     kw = {"nxyz": 64} if name == "skin_vessels" else {}
     sc, _ = scenes.build(name, **kw)
@@ -354,10 +354,12 @@ def test_per_xcd_partial_grids_match_shared_grid(gpu, name):
     a = eng.run(300_000, seed=41)
     a2 = eng.run(300_000, seed=41)
     b = eng.run(300_000, seed=41, flags=A.FLAG_SHARED_GRID)
+    # four partial grids per XCD (SMCRT_FLAG_GRID_COPIES): the same again
+    a4 = eng.run(300_000, seed=41, flags=A.flag_grid_copies(4))
     for k in ("absorbed", "escaped_r", "escaped_t", "killed_surface", "scatters", "ev_crossings", "ev_atomics"):
-        assert a.stats[k] == b.stats[k] == a2.stats[k], k
+        assert a.stats[k] == b.stats[k] == a2.stats[k] == a4.stats[k], k
     m = b.jmean > 1e-3 * b.jmean.max()
-    for x in (a, a2):
+    for x in (a, a2, a4):
         assert abs(x.jmean.sum() / b.jmean.sum() - 1) < 1e-10
         assert np.max(np.abs(x.jmean[m] / b.jmean[m] - 1)) < 1e-9
 

@@ -33,6 +33,7 @@ FLAG_SHARED_GRID = 1 << 5
 FLAG_DIRECT_TALLY = 1 << 6
 FLAG_BINNED_TALLY = 1 << 7
 FLAG_GRID_COPIES_SHIFT = 8  # flags bits 8-10: log2(partial grids per XCD) + 1, 0 = library default
+FLAG_GRID_COPIES_MASK = 7 << FLAG_GRID_COPIES_SHIFT
 
 
 def flag_grid_copies(per_xcd: int) -> int:

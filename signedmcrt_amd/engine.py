@@ -89,12 +89,16 @@ class Engine:
             pass
 
     def _shape(self, flags: int, shape: dict):
-        """Explicit keywords win over the engine's tuned shape; flags combine."""
+        """Explicit keywords win over the engine's tuned shape; flags combine (an explicit partial-grid
+        count replaces the tuned one)."""
         if not self.shape:
             return flags, shape
         from .autotune import run_kwargs
         kw = run_kwargs(self.shape)
-        flags |= kw.pop("flags")
+        tuned = kw.pop("flags")
+        if flags & A.FLAG_GRID_COPIES_MASK:
+            tuned &= ~A.FLAG_GRID_COPIES_MASK
+        flags |= tuned
         kw.update(shape)
         return flags, kw
 

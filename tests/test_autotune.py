@@ -63,6 +63,17 @@ def test_run_kwargs_maps_grid_copies_to_flag():
         A.flag_grid_copies(3)
 
 
+def test_explicit_grid_copies_replace_the_tuned_ones():
+    from signedmcrt_amd.engine import Engine
+    e = Engine.__new__(Engine)
+    e.shape = {"copies": 8, "tally": "direct"}
+    flags, kw = e._shape(A.flag_grid_copies(2), {})
+    assert flags & A.FLAG_GRID_COPIES_MASK == A.flag_grid_copies(2)
+    assert flags & A.FLAG_DIRECT_TALLY
+    flags, kw = e._shape(0, {})
+    assert flags & A.FLAG_GRID_COPIES_MASK == A.flag_grid_copies(8)
+
+
 def test_tuned_shape_runs_a_batch():
     """bench.py's batched workload hands the tuner's shape to Batch, which builds its launch
     options with engine.run_opts: every knob the tuner returns (adjoint waves included) must map

@@ -71,13 +71,31 @@ def _nested_polygons_scene(square_first: bool):
     return Scene(arr, cart_grid(40, 40, 8, 1.2, 1.2, 0.25), src)
 
 
+def _moved_polygons_scene():
+    """Two polygons in different frames (one translated, one rotated about z): the compiled
+    kernels' gathered evaluation takes each lane's first containing rectangle in its own frame,
+    and the disjoint-rectangle shortcut does not apply."""
+    from signedmcrt_amd.sdf import extrude, invert, polygon, rotate_z, translate
+    tri = np.array([(-0.6, -0.5), (0.7, -0.4), (0.1, 0.6)])
+    quad = np.array([(-0.5, -0.3), (0.5, -0.3), (0.5, 0.3), (-0.5, 0.3)])
+    arr = [extrude(polygon(tri, mono(50.0, 0.4, 0.85, 1.4), 1, transform=invert(translate((-0.4, 0.3, 0.0)))), 0.25),
+           extrude(polygon(quad, mono(50.0, 1.2, 0.85, 1.4), 2, transform=rotate_z(0.5)), 0.25)]
+    arr.append(box((2.4, 2.4, 0.5), mono(50.0, 0.2, 0.85, 1.4), 3))
+    arr.append(box((4.4, 4.4, 2.5), mono(0.0, 0.0, 0.0, 1.0), 4))
+    src = source("uniform", direction=(0.0, 0.0, -1.0), point1=(-1.2, -1.2, 0.25 - 1e-4),
+                 point2=(2.4, 0.0, 0.0), point3=(0.0, 2.4, 0.0))
+    return Scene(arr, cart_grid(40, 40, 8, 1.2, 1.2, 0.25), src)
+
+
 @pytest.mark.parametrize("name", ["refracting_sphere", "skin_vessels", "polygon_regions", "zoo",
-                                  "nested_polygons", "nested_polygons_square_first"])
+                                  "nested_polygons", "nested_polygons_square_first", "moved_polygons"])
 def test_compiled_kernels_match_interpreter(gpu, name, monkeypatch, tmp_path):
     if name == "zoo":
         sc = _zoo_scene()
     elif name.startswith("nested_polygons"):
         sc = _nested_polygons_scene(name.endswith("square_first"))
+    elif name == "moved_polygons":
+        sc = _moved_polygons_scene()
     else:
         kw = {"nxyz": 32} if name == "skin_vessels" else ({"n_side": 2, "n_edges": 40, "nxy": 40, "nz": 8}
                                                           if name == "polygon_regions" else {})

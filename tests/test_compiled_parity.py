@@ -1,7 +1,7 @@
 """Oracle parity of the packet kernel compiled at run time for the scene (SMCRT_JIT=1, built for 4
 resident waves per SIMD: the kernel the benchmark times) on the slab (1e6-packet config, 100^3
-grid), the refracting sphere (200^3) and skin + vessels (256^3) workloads at their own grids, and
-of its adjoint on skin + vessels.
+grid), the refracting sphere (200^3) and skin + vessels (256^3) workloads at their own grids, on
+a four-polygon version of the polygon workload, and of its adjoint on skin + vessels.
 
 Bars:
   - absorbed / reflected / transmitted / killed fractions: two-sample z within 3;
@@ -11,7 +11,7 @@ Bars:
     A pencil beam's fluence falls by four decades within a few voxels of the axis (466 voxels of
     the slab and 150 of skin + vessels hold >= 1 % of the maximum), so the floor is set per
     workload to bring >= 1e4 voxels in: 1e-3 of the maximum for the slab, 3e-4 for skin +
-    vessels, 1e-2 for the refracting sphere's broad beam.  The same test run oracle against oracle
+    vessels, 1e-2 for the refracting sphere's broad beam and the polygons' uniform source.  The same test run oracle against oracle
     (two seeds, these batch sizes) gives 0.2-0.5 % beyond |z| = 3 and |mean z| <= 0.035;
   - the per-packet fluence standard deviation (median over the same voxels of sd_gpu/sd_oracle,
     each from its batches) within 3 % of 1: the two estimators have the same variance, so
@@ -37,7 +37,11 @@ CASES = {
     "slab": (500_000, 50_000, 1e-3),
     "refracting_sphere": (250_000, 25_000, 1e-2),
     "skin_vessels": (1_000_000, 50_000, 3e-4),
+    # four polygons of 40 edges (the gathered polygon evaluation, disjoint rectangles) on an
+    # 80 x 80 x 10 grid: the oracle evaluates each polygon by brute force
+    "polygon_regions": (500_000, 50_000, 1e-2),
 }
+BUILD_KW = {"polygon_regions": {"n_side": 2, "n_edges": 40, "nxy": 80, "nz": 10}}
 SHAPE = {"waves": 4}
 FRACTIONS = ("absorbed", "escaped_r", "escaped_t", "killed_surface")
 
@@ -105,7 +109,7 @@ def _oracle_batches(sc, n, seed):
 @pytest.mark.parametrize("name", list(CASES))
 def test_compiled_kernel_matches_oracle(gpu, name, monkeypatch, tmp_path):
     n_gpu, n_orc, floor = CASES[name]
-    sc, _ = scenes.build(name)
+    sc, _ = scenes.build(name, **BUILD_KW.get(name, {}))
     eng = _compiled_engine(sc, monkeypatch, tmp_path)
     mg, tg = _gpu_batches(eng, sc, n_gpu, seed=2025)
     kname = eng.kernel_variant()[0]

@@ -11,8 +11,9 @@ Bars:
     A pencil beam's fluence falls by four decades within a few voxels of the axis (466 voxels of
     the slab and 150 of skin + vessels hold >= 1 % of the maximum), so the floor is set per
     workload to bring >= 1e4 voxels in: 1e-3 of the maximum for the slab, 3e-4 for skin +
-    vessels, 1e-2 for the refracting sphere's broad beam and the polygons' uniform source.  The same test run oracle against oracle
-    (two seeds, these batch sizes) gives 0.2-0.5 % beyond |z| = 3 and |mean z| <= 0.035;
+    vessels, 1e-2 for the refracting sphere's broad beam and the polygons' uniform source.  The
+    same test run oracle against oracle (two seeds, these batch sizes) gives 0.2-0.5 % beyond
+    |z| = 3 and |mean z| <= 0.035;
   - the per-packet fluence standard deviation (median over the same voxels of sd_gpu/sd_oracle,
     each from its batches) within 3 % of 1: the two estimators have the same variance, so
     packets/s compares the two at equal fluence error (oracle against oracle: within 0.3 %);

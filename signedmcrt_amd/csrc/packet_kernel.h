@@ -819,6 +819,48 @@ struct InterpretedSweep {
     static SMCRT_DEV float region(int, const V3<float>&, const float* __restrict__, const PolyHit&) { return 0.f; }
 };
 
+// Adds the outcome counts of the lanes active here (divergent code: the lanes whose set changes) to
+// their old set's counters (stats[set * NOUT + i]) and clears them.  For each distinct set among
+// them, each counter is summed over its lanes in scalar registers (readlane) and added by one
+// atomic: atomics per lane on the same few counters serialise in L2, and the lanes cross each set
+// boundary of a batch at nearly the same time.
+SMCRT_DEV void flush_outcomes(unsigned long long* stats, int set, unsigned (&out)[NOUT], int lane) {
+    uint64_t act = __builtin_amdgcn_read_exec();
+    while (act) {
+        const int lead = __builtin_ctzll(act);
+        const int s0 = __builtin_amdgcn_readlane(set, lead);
+        const uint64_t grp = wave_ballot(set == s0) & act;
+        act &= ~grp;
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) {
+            unsigned long long sum = 0;
+            for (uint64_t g = grp; g; g &= g - 1)
+                sum += (unsigned)__builtin_amdgcn_readlane((int)out[i], __builtin_ctzll(g));
+            if (lane == lead && sum) atomicAdd(stats + (size_t)s0 * NOUT + i, sum);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) out[i] = 0u;
+}
+
+// The lanes' outcome counts at the end of the launch: for each distinct set among the wavefront's
+// lanes, each counter summed over that set's lanes (shuffles) and added by one lane.  Call
+// converged.
+SMCRT_DEV void flush_outcomes_wave(unsigned long long* stats, int set, const unsigned (&out)[NOUT], int lane) {
+    uint64_t todo = ~0ull;
+    while (todo) {
+        const int lead = __builtin_ctzll(todo);
+        const int s0 = __shfl(set, lead);
+        const bool mine = set == s0;
+        todo &= ~wave_ballot(mine);
+        for (int i = 0; i < NOUT; ++i) {
+            unsigned long long v = mine ? (unsigned long long)out[i] : 0ull;
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+            if (lane == lead && v) atomicAdd(stats + (size_t)s0 * NOUT + i, v);
+        }
+    }
+}
+
 // WAVES: resident waves per SIMD the registers are allocated for (2: at most 256 VGPRs, 3: at most
 // 168).  SW: the sweep policy.  BIN: the binned tally's stage (forward single-set launches of
 // the kernels compiled at run time; KParams).
@@ -902,7 +944,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     unsigned ev[NEV];  // event counters: wave-uniform (ballot counts at converged points)
 #pragma unroll
     for (int i = 0; i < NEV; ++i) ev[i] = 0;
-    int cur_set = 0;
 
     unsigned long long base = 0;
     unsigned avail = 0;
@@ -947,14 +988,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                         s = lo;
                         pid = C->set_first[s] + (w - C->set_prefix[s]);
                     }
-                    if (s != cur_set) {
-#pragma unroll
-                        for (int i = 0; i < NOUT; ++i)
-                            if (out[i]) atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
-#pragma unroll
-                        for (int i = 0; i < NOUT; ++i) out[i] = 0;
-                        cur_set = s;
-                    }
+                    // per-lane counters belong to the lane's set: flushed when it changes
+                    if (s != L.set) flush_outcomes(C->stats, L.set, out, lane_id);
                     L.set = s;
                     L.pid = pid;
                     L.st = ST_EMIT;
@@ -971,7 +1006,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
         }
 
         const DevMat* mt = s_mat + L.set * P.nreg;
-        double* __restrict__ grid = P.jmean ? P.jmean + (size_t)L.set * P.grid_stride : nullptr;
+        const bool grid = P.jmean != nullptr;  // tallying (wave-uniform)
         const uint32_t gbase = (uint32_t)L.set * (uint32_t)P.grid_stride;
 
         // ---- emission ----
@@ -994,7 +1029,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             L.iters = 0;
             L.ghost = false;
             if constexpr (kWeight) L.weight = 1.f;
-            if constexpr (!kWaveOut) out[O_LAUNCHED]++;
             if (ADJ) {
                 adjC = 0.f;
                 for (int w = 0; w < 4; ++w)
@@ -1262,7 +1296,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             ev[E_SWEEPS] -= __builtin_popcountll(wave_ballot(absorbed));
         }
         if constexpr (kWaveOut) out[O_SCATTERS] += __builtin_popcountll(wave_ballot(scat));
-        else out[O_SCATTERS] += scat ? 1u : 0u;
         if (kWeight && P.jmean) ev[E_ATOMICS] += tally_push(Q, P, wpush, widx, wlen, tbase, lane_id);
 
         // ---- march step (src/inttau2.f90:64-83): optical depth, then the voxel walk ----
@@ -1412,8 +1445,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
             pend_idx = -1;
             pend_li = -1;
             if (kDet && C->n_det > 0 && end_kind != END_ERR) record_detectors(C, L);
-            // outcome counters as selects (error paths are counted where they occur)
+            // per-lane outcome counters, once per packet (its launch and scatters too: counting
+            // in the event loop, per lane, costs the sets kernels a third of their time)
             if constexpr (!kWaveOut) {
+                out[O_LAUNCHED] += 1u;
+                out[O_SCATTERS] += (unsigned)L.nsc;
                 out[O_ABSORBED] += end_kind == END_ABSORB ? 1u : 0u;
                 out[O_R] += end_kind == END_R ? 1u : 0u;
                 out[O_T] += end_kind == END_T ? 1u : 0u;
@@ -1486,10 +1522,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     }
 
     // ---- flush counters ----
+    if constexpr (kWaveOut) {
 #pragma unroll
-    for (int i = 0; i < NOUT; ++i)
-        if (out[i] && (!kWaveOut || lane_id == 0))
-            atomicAdd(C->stats + (size_t)cur_set * NOUT + i, (unsigned long long)out[i]);
+        for (int i = 0; i < NOUT; ++i)
+            if (out[i] && lane_id == 0) atomicAdd(C->stats + i, (unsigned long long)out[i]);
+    } else {
+        flush_outcomes_wave(C->stats, L.set, out, lane_id);
+    }
     unsigned long long* evg = C->stats + (size_t)P.nsets * NOUT;
 #pragma unroll
     for (int i = 0; i < NEV; ++i)

@@ -418,7 +418,12 @@ def bench_batched(args):
         res = engines[1].tune(use_cache=False, adjoint=False)
         shape = batch_shape(res.shape)
         tuned = {"shape": res.shape, "tuned_on": "refracting_sphere 64^3", "speedup_vs_default": res.speedup}
-    b = Batch(engines, sets, seed=123456789, **shape)
+    if args.no_tune:
+        b = Batch(engines, sets, seed=123456789, **shape)
+    else:
+        # the batch's own wave count (its kernels' registers differ from a single launch's)
+        b = Batch.fastest(engines, sets, seed=123456789, **{k: v for k, v in shape.items() if k != "waves"})
+        tuned["batch_waves"] = b.shape["waves"]
     for _ in range(args.warmup):
         b.run()
     torch.cuda.synchronize()
@@ -453,7 +458,7 @@ def bench_batched(args):
     achieved = flops / (replay_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-            "kernel": " + ".join(batch_kernel_label(eng, int(shape.get("waves", 3)),
+            "kernel": " + ".join(batch_kernel_label(eng, int(b.shape.get("waves", 3)),
                                                     sum(s.n_packets for s in sets if s.scene == geo))
                                  for geo, eng in enumerate(engines)) + " (one graph replay)",
             "kernel_ms": replay_ms, "flops_per_launch": flops}
@@ -494,7 +499,7 @@ def bench_batched(args):
                    "budget_min": int(budgets.min()), "budget_max": int(budgets.max()),
                    "grids": "64^3 FP64 per set", "geometries": ["slab", "refracting_sphere"],
                    "launch": "one HIP graph: fill kernels + one persistent packet-kernel node per geometry",
-                   "launch_shape": shape or "library default"},
+                   "launch_shape": b.shape or "library default"},
         "fractions": frac, "autotune": tuned, "roofline": roof, "roofline_tally": roof_tally,
         "cpu_baseline": cpu,
     }

@@ -69,6 +69,34 @@ class Batch:
                                            C.byref(st)))
         return jm.reshape(g.shape, order="F"), st.as_dict()
 
+    @classmethod
+    def fastest(cls, engines: List[Engine], sets: List[BatchSet], seed: int = 123456789,
+                waves: Sequence[int] = (3, 4), **shape) -> "Batch":
+        """The batch at the wave count (resident waves per SIMD the kernel's registers are
+        allocated for) whose replay is fastest, one timed replay each after a warm one.  A shape
+        tuned on single launches does not carry over: the kernels of several sets keep per-lane
+        set state and outcome counters, and 4 waves (128 VGPRs) can win where a single launch
+        runs best at 3."""
+        best, best_ms = None, None
+        for w in waves:
+            b = cls(engines, sets, seed, **dict(shape, waves=int(w)))
+            b.run()
+            b.run()
+            ms = b.replay_ms()
+            if best is None or ms < best_ms:
+                if best is not None:
+                    best.close()
+                best, best_ms = b, ms
+            else:
+                b.close()
+        return best
+
+    def replay_ms(self) -> float:
+        """Device time of the last replay (ms)."""
+        st = A.Stats()
+        A.check(self._lib.smcrt_batch_read(self.handle, 0, None, C.byref(st)))
+        return float(st.kernel_ms)
+
     def close(self) -> None:
         if self.handle:
             self._lib.smcrt_batch_destroy(self.handle)

@@ -93,6 +93,27 @@ def test_graph_replays_repeat(gpu):
             np.testing.assert_allclose(jm, first[i][0], rtol=1e-9, atol=1e-12)
 
 
+def test_fastest_wave_count_keeps_results(gpu, monkeypatch, tmp_path):
+    """Batch.fastest keeps the batch at the faster of 3 and 4 waves per SIMD (the compiled kernels
+    of several sets): its counters equal those of the batch at the other wave count, its grids
+    within FP64 summation order."""
+    monkeypatch.setenv("SMCRT_JIT", "1")
+    monkeypatch.setenv("SMCRT_CACHE_DIR", str(tmp_path))
+    engines = _engines()
+    sets = _sets(engines, np.random.default_rng(9), 6)
+    b = Batch.fastest(engines, sets, seed=31)
+    assert b.shape["waves"] in (3, 4)
+    assert b.replay_ms() > 0.0
+    other = Batch(engines, sets, seed=31, waves=7 - b.shape["waves"])
+    other.run()
+    for i in range(len(sets)):
+        jm, st = b.read(i)
+        jo, so = other.read(i)
+        for k in OUT:
+            assert st[k] == so[k], (i, k, st[k], so[k])
+        np.testing.assert_allclose(jm, jo, rtol=1e-9, atol=1e-12)
+
+
 # ---- the batched workload (256 seeded property sets) against the oracle -----------------------
 def _workload_sets(n_sets, cap=None):
     """Property sets of the batched benchmark (bench.batched_sets): even sets on the slab, odd on

@@ -948,8 +948,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     unsigned long long base = 0;
     unsigned avail = 0;
     bool exhausted = false;
+    // the sets' queue positions and first packet ids, read-only here: scalar loads where the index
+    // is wave-uniform
+    typedef __attribute__((address_space(4))) const unsigned long long CU64;
+    CU64* const sprefix = (CU64*)C->set_prefix;
+    CU64* const sfirst = (CU64*)C->set_first;
     // one set: packet id = first id + queue position (read once, scalar)
-    const unsigned long long first0 = C->set_first[0] - C->set_prefix[0];
+    const unsigned long long first0 = sfirst[0] - sprefix[0];
+    // several sets: the wavefront's current set cs holds queue positions [cs_lo, cs_hi) from packet
+    // id cs_first on; it moves (a scalar binary search) when a claimed chunk starts past it, and
+    // only lanes of a chunk that straddles a set boundary search on their own (vector loads, which
+    // wait for the wavefront's tally atomics in flight: once per refill they cost batches a quarter
+    // of their time)
+    int cs = 0;
+    unsigned long long cs_lo = 0, cs_hi = 0, cs_first = 0;
     const float eps = P.eps;
     const float eps_kill = P.eps_kill;
     const unsigned chunk = (unsigned)P.chunk;
@@ -967,7 +979,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     unsigned long long b = 0;
                     if (lane_id == 0) b = atomicAdd(P.work, (unsigned long long)chunk);
                     b = __shfl(b, 0);
+                    b = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(b >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);  // wave-uniform
                     if (b >= P.total) { exhausted = true; break; }
+                    if (P.nsets > 1 && b >= cs_hi) {
+                        int lo = 0, hi = P.nsets - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (sprefix[mid] <= b) lo = mid; else hi = mid - 1;
+                        }
+                        cs = lo;
+                        cs_lo = sprefix[lo];
+                        cs_hi = sprefix[lo + 1];
+                        cs_first = sfirst[lo];
+                    }
                     base = b;
                     unsigned long long rem = P.total - b;
                     avail = rem < chunk ? (unsigned)rem : chunk;
@@ -980,13 +1005,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
                     int s = 0;
                     unsigned long long pid = first0 + w;
                     if (P.nsets > 1) {
-                        int lo = 0, hi = P.nsets - 1;
-                        while (lo < hi) {
-                            int mid = (lo + hi + 1) >> 1;
-                            if (C->set_prefix[mid] <= w) lo = mid; else hi = mid - 1;
+                        if (w >= cs_lo && w < cs_hi) {
+                            s = cs;
+                            pid = cs_first + (w - cs_lo);
+                        } else {
+                            int lo = 0, hi = P.nsets - 1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (sprefix[mid] <= w) lo = mid; else hi = mid - 1;
+                            }
+                            s = lo;
+                            pid = sfirst[s] + (w - sprefix[s]);
                         }
-                        s = lo;
-                        pid = C->set_first[s] + (w - C->set_prefix[s]);
                     }
                     // per-lane counters belong to the lane's set: flushed when it changes
                     if (s != L.set) flush_outcomes(C->stats, L.set, out, lane_id);

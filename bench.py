@@ -384,9 +384,11 @@ def batched_sets(n_sets: int, seed: int = 20240917):
 
 
 def batch_shape(shape):
-    """The launch-shape knobs of a tuned shape that a batch takes: the schedule only (batches
-    tally straight to HBM: no LDS tile, no binned tally; no adjoint)."""
-    return {k: v for k, v in shape.items() if k in ("block", "chunk", "walk_cap", "refill_min", "waves")}
+    """The launch-shape knobs of a tuned single-launch shape that a batch takes: block size and
+    walk cap (batches tally straight to HBM: no LDS tile, no binned tally; no adjoint; their wave
+    count, chunk and refill policy are their own, `Batch.fastest` and the library's batch
+    defaults)."""
+    return {k: v for k, v in shape.items() if k in ("block", "walk_cap")}
 
 
 def bench_batched(args):
@@ -421,9 +423,9 @@ def bench_batched(args):
     if args.no_tune:
         b = Batch(engines, sets, seed=123456789, **shape)
     else:
-        # the batch's own wave count (its kernels' registers differ from a single launch's)
-        b = Batch.fastest(engines, sets, seed=123456789, **{k: v for k, v in shape.items() if k != "waves"})
-        tuned["batch_waves"] = b.shape["waves"]
+        # the batch's own wave count and refill policy (Batch.fastest's candidates)
+        b = Batch.fastest(engines, sets, seed=123456789, **shape)
+        tuned["batch_shape"] = b.shape
     for _ in range(args.warmup):
         b.run()
     torch.cuda.synchronize()

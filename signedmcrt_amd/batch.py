@@ -71,15 +71,17 @@ class Batch:
 
     @classmethod
     def fastest(cls, engines: List[Engine], sets: List[BatchSet], seed: int = 123456789,
-                waves: Sequence[int] = (3, 4), **shape) -> "Batch":
-        """The batch at the wave count (resident waves per SIMD the kernel's registers are
-        allocated for) whose replay is fastest, one timed replay each after a warm one.  A shape
-        tuned on single launches does not carry over: the kernels of several sets keep per-lane
-        set state and outcome counters, and 4 waves (128 VGPRs) can win where a single launch
-        runs best at 3."""
+                candidates: Optional[Sequence[dict]] = None, **shape) -> "Batch":
+        """The batch whose replay is fastest among launch-shape candidates (default: 3 and 4
+        resident waves per SIMD, refills once 8 or 16 lanes are idle), one timed replay each after
+        a warm one; `shape` holds the knobs common to all.  A shape tuned on single launches does
+        not carry over: the kernels of several sets keep per-lane set state and outcome counters
+        (4 waves can win where a single launch runs best at 3), and their refills cost more."""
+        if candidates is None:
+            candidates = [dict(waves=w, refill_min=r) for w in (3, 4) for r in (8, 16)]
         best, best_ms = None, None
-        for w in waves:
-            b = cls(engines, sets, seed, **dict(shape, waves=int(w)))
+        for c in candidates:
+            b = cls(engines, sets, seed, **dict(shape, **c))
             b.run()
             b.run()
             ms = b.replay_ms()

@@ -75,9 +75,10 @@ def test_explicit_grid_copies_replace_the_tuned_ones():
 
 
 def test_tuned_shape_runs_a_batch():
-    """bench.py's batched workload hands the tuner's shape to Batch, which builds its launch
-    options with engine.run_opts: every knob the tuner returns (adjoint waves included) must map
-    to an accepted keyword after batch_shape."""
+    """bench.py's batched workload hands part of the tuner's shape to Batch, which builds its
+    launch options with engine.run_opts: the knobs batch_shape keeps (block size, walk cap) map to
+    accepted keywords, together with Batch.fastest's candidates (waves, refill policy); chunk and
+    refill policy stay unset (0) for the library's batch defaults."""
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -85,9 +86,13 @@ def test_tuned_shape_runs_a_batch():
     from signedmcrt_amd.engine import run_opts
     shape = {k: v[-1] for k, v in autotune.SPACE.items()}
     shape["adj_waves"] = 4
-    o = run_opts(0, **bench.batch_shape(shape))
-    assert o.block_size == shape["block"] and o.chunk == shape["chunk"] and o.walk_cap == shape["walk_cap"]
-    assert o.refill_min == shape["refill_min"] and o.waves == shape["waves"]
+    kept = bench.batch_shape(shape)
+    assert set(kept) == {"block", "walk_cap"}
+    o = run_opts(0, **kept)
+    assert o.block_size == shape["block"] and o.walk_cap == shape["walk_cap"]
+    assert o.chunk == 0 and o.refill_min == 0
+    o = run_opts(0, **dict(kept, waves=4, refill_min=16))
+    assert o.waves == 4 and o.refill_min == 16
 
 
 def test_fingerprint_tracks_the_scene():

@@ -73,12 +73,12 @@ class Batch:
     def fastest(cls, engines: List[Engine], sets: List[BatchSet], seed: int = 123456789,
                 candidates: Optional[Sequence[dict]] = None, **shape) -> "Batch":
         """The batch whose replay is fastest among launch-shape candidates (default: 3 and 4
-        resident waves per SIMD, refills once 8 or 16 lanes are idle), one timed replay each after
+        resident waves per SIMD, refills once 4 or 8 lanes are idle), one timed replay each after
         a warm one; `shape` holds the knobs common to all.  A shape tuned on single launches does
         not carry over: the kernels of several sets keep per-lane set state and outcome counters
         (4 waves can win where a single launch runs best at 3), and their refills cost more."""
         if candidates is None:
-            candidates = [dict(waves=w, refill_min=r) for w in (3, 4) for r in (8, 16)]
+            candidates = [dict(waves=w, refill_min=r) for w in (3, 4) for r in (4, 8)]
         best, best_ms = None, None
         for c in candidates:
             b = cls(engines, sets, seed, **dict(shape, **c))

@@ -2254,12 +2254,18 @@ extern "C" int smcrt_batch_create(smcrt_scene* const* scenes, int32_t n_scenes, 
         // large groups run the scene's compiled kernel for several sets (same transport, the
         // program as straight-line code)
         const bool compiled = batch_jit_wanted(shape, acc) && jit_build(s);
-        // batches refill a wavefront from chunks of 256 queue positions once 8 of its lanes are
-        // idle, unless the shape says otherwise (256-set workload: 3276 -> 2592 ms per replay
-        // against the single-launch defaults, 64 and every idle lane)
+        // Batches refill a wavefront once 8 of its lanes are idle, from chunks as large as give
+        // every wavefront about 32 claims (64 to 2048 queue positions), unless the shape says
+        // otherwise: a claim's returning atomic waits for the wavefront's tally atomics in flight,
+        // which batches (no LDS tile) keep many of.  256-set workload: 3276 ms per replay with the
+        // single-launch defaults (64, every idle lane), 2592 with 256 and 8, 2392 with 2048 and 8.
         smcrt_run_opts bo{};
         if (shape) bo = *shape;
-        if (bo.chunk <= 0) bo.chunk = 256;
+        if (bo.chunk <= 0) {
+            const unsigned long long per_wave = acc / ((unsigned long long)std::max(s->num_cu, 1) * 16ull * 32ull);
+            bo.chunk = 64;
+            while (bo.chunk < 2048 && 2ull * (unsigned long long)bo.chunk <= per_wave) bo.chunk *= 2;
+        }
         if (bo.refill_min <= 0) bo.refill_min = 8;
         g.sh = pick_shape(s, &bo, false, (int)g.sets.size(), 0, 0, compiled, 0, compiled);
         P.chunk = g.sh.chunk;
